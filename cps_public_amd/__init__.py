@@ -1,0 +1,237 @@
+"""MI355X-native CPS Wilson / domain-wall even-odd preconditioned operators and CG.
+
+The compute path is libcps_hip.so (HIP kernels for gfx950 behind the C ABI in
+include/cps_hip.h).  This module loads it with ctypes and offers a thin
+Python mirror of the CPS entry points for host-side callers and tests.
+Importing works without a GPU; any call that needs the device raises
+CpsHipError when the library or the device is missing -- there is no CPU
+fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcps_hip.so")
+HEADER_PATH = os.path.join(_HERE, "..", "include", "cps_hip.h")
+
+OP_WILSON, OP_DWF, OP_MOBIUS, OP_CLOVER = 0, 1, 2, 3
+PREC_DOUBLE, PREC_MIXED_HALF, PREC_MIXED_BF16, PREC_MIXED_SINGLE = 0, 1, 2, 3
+ERR_NAMES = {-1: "bad argument", -2: "HIP runtime failure", -3: "no gauge field resident",
+             -4: "CG d == 0 (inv_cg.C:326-329)", -5: "device allocation failed", -6: "no HIP device"}
+
+
+class CpsHipError(RuntimeError):
+    def __init__(self, code, what=""):
+        super().__init__(f"{what}: cps_hip error {code} ({ERR_NAMES.get(code, 'unknown')})")
+        self.code = code
+
+
+class Op(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("mass", ctypes.c_double), ("m5", ctypes.c_double),
+                ("a5_inv", ctypes.c_double), ("mobius_b", ctypes.c_double), ("mobius_c", ctypes.c_double),
+                ("csw", ctypes.c_double)]
+
+
+class CgArg(ctypes.Structure):
+    _fields_ = [("max_num_iter", ctypes.c_int), ("stop_rsd", ctypes.c_double), ("precision", ctypes.c_int),
+                ("reliable_every", ctypes.c_int), ("reliable_delta", ctypes.c_double),
+                ("check_every", ctypes.c_int)]
+
+
+class CgResult(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("true_res", ctypes.c_double),
+                ("src_norm_sq", ctypes.c_double), ("reliable_updates", ctypes.c_int), ("seconds", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcps_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CpsHipError(-2, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(l):
+    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    P = ctypes.POINTER
+    sig = {
+        "cps_hip_lattice_create": [P(vp), vp, i, vp],
+        "cps_hip_lattice_destroy": [vp],
+        "cps_hip_gauge_import": [vp, vp],
+        "cps_hip_wilson_dslash": [vp, vp, vp, vp, i, i],
+        "cps_hip_wilson_m": [vp, vp, vp, vp, d],
+        "cps_hip_wilson_mdag": [vp, vp, vp, vp, d],
+        "cps_hip_wilson_mdagm": [vp, vp, vp, vp, vp, d],
+        "cps_hip_dwf_dslash": [vp, vp, vp, vp, d, i, i, d, d],
+        "cps_hip_dwf_m": [vp, vp, vp, vp, d, d, d],
+        "cps_hip_dwf_mdag": [vp, vp, vp, vp, d, d, d],
+        "cps_hip_dwf_mdagm": [vp, vp, vp, vp, vp, d, d, d],
+        "cps_hip_matpc_dag_matpc": [vp, P(Op), vp, vp, vp, i],
+        "cps_hip_matpc": [vp, P(Op), vp, vp, i, i],
+        "cps_hip_matpc_dag_matpc_prec": [vp, P(Op), i, vp, vp, vp, i],
+        "cps_hip_clover_build": [vp, vp, i, d],
+        "cps_hip_clover_export": [vp, i, i, vp],
+        "cps_hip_invcg": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
+        "cps_hip_matinv": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
+        "cps_hip_bench_hop": [vp, i, i, i, i, P(d), P(d)],
+        "cps_hip_device_count": [],
+    }
+    for name, args in sig.items():
+        f = getattr(l, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    l.cps_hip_version.restype = ctypes.c_char_p
+    l.cps_hip_lattice_device_bytes.argtypes = [vp]
+    l.cps_hip_lattice_device_bytes.restype = ctypes.c_longlong
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise CpsHipError(rc, what)
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def make_op(kind, mass=0.0, m5=1.8, a5_inv=1.0, b=1.0, c=0.0, csw=0.0):
+    return Op(kind, mass, m5, a5_inv, b, c, csw)
+
+
+def make_cg_arg(stop_rsd=1e-10, max_num_iter=10000, precision=PREC_DOUBLE, reliable_every=100,
+                reliable_delta=0.1, check_every=0):
+    return CgArg(max_num_iter, stop_rsd, precision, reliable_every, reliable_delta, check_every)
+
+
+class Lattice:
+    """Device handle: geometry, resident gauge field and scratch (wilson_init /
+    dwf_init equivalent).  node_sites = (Lx, Ly, Lz, Lt)."""
+
+    def __init__(self, node_sites, ls=1, bc=(0, 0, 0, 1)):
+        self.L = [int(v) for v in node_sites]
+        self.ls = int(ls)
+        self.bc = [int(v) for v in bc]
+        h = ctypes.c_void_p()
+        dims = (ctypes.c_int * 4)(*self.L)
+        bcs = (ctypes.c_int * 4)(*self.bc)
+        _check(lib().cps_hip_lattice_create(ctypes.byref(h), dims, self.ls, bcs), "cps_hip_lattice_create")
+        self.h = h
+
+    @property
+    def vcb(self):
+        return self.L[0] * self.L[1] * self.L[2] * self.L[3] // 2
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cps_hip_lattice_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def import_gauge(self, u_wilson):
+        _check(lib().cps_hip_gauge_import(self.h, _ptr(u_wilson)), "cps_hip_gauge_import")
+
+    def device_bytes(self):
+        return int(lib().cps_hip_lattice_device_bytes(self.h))
+
+    # ---- Wilson library ----
+    def wilson_dslash(self, psi, cb, dag, u=None):
+        chi = np.zeros(24 * self.vcb)
+        _check(lib().cps_hip_wilson_dslash(self.h, _ptr(chi), None if u is None else _ptr(u), _ptr(psi), cb, dag),
+               "cps_hip_wilson_dslash")
+        return chi
+
+    def wilson_m(self, psi, kappa, dag=0):
+        chi = np.zeros(24 * self.vcb)
+        f = lib().cps_hip_wilson_mdag if dag else lib().cps_hip_wilson_m
+        _check(f(self.h, _ptr(chi), None, _ptr(psi), kappa), "cps_hip_wilson_m")
+        return chi
+
+    def wilson_mdagm(self, psi, kappa):
+        chi = np.zeros(24 * self.vcb)
+        mp = np.zeros(1)
+        _check(lib().cps_hip_wilson_mdagm(self.h, _ptr(chi), None, _ptr(psi), _ptr(mp), kappa), "cps_hip_wilson_mdagm")
+        return chi, float(mp[0])
+
+    # ---- DWF library ----
+    def dwf_dslash(self, inp, mass, cb, dag, m5=1.8, a5_inv=1.0):
+        out = np.zeros(24 * self.vcb * self.ls)
+        _check(lib().cps_hip_dwf_dslash(self.h, _ptr(out), None, _ptr(inp), mass, cb, dag, m5, a5_inv),
+               "cps_hip_dwf_dslash")
+        return out
+
+    def dwf_mdagm(self, inp, mass, m5=1.8, a5_inv=1.0):
+        out = np.zeros(24 * self.vcb * self.ls)
+        dp = np.zeros(1)
+        _check(lib().cps_hip_dwf_mdagm(self.h, _ptr(out), None, _ptr(inp), _ptr(dp), mass, m5, a5_inv),
+               "cps_hip_dwf_mdagm")
+        return out, float(dp[0])
+
+    # ---- generic operator entry points ----
+    def _nreal(self, op):
+        return 24 * self.vcb * (1 if op.kind in (OP_WILSON, OP_CLOVER) else self.ls)
+
+    def mdagm(self, op, inp, nrhs=1):
+        out = np.zeros(self._nreal(op) * nrhs)
+        dp = np.zeros(nrhs)
+        _check(lib().cps_hip_matpc_dag_matpc(self.h, ctypes.byref(op), _ptr(out), _ptr(inp), _ptr(dp), nrhs),
+               "cps_hip_matpc_dag_matpc")
+        return out, dp
+
+    def mdagm_prec(self, op, inp, precision, nrhs=1):
+        out = np.zeros(self._nreal(op) * nrhs)
+        dp = np.zeros(nrhs)
+        _check(lib().cps_hip_matpc_dag_matpc_prec(self.h, ctypes.byref(op), precision, _ptr(out), _ptr(inp),
+                                                  _ptr(dp), nrhs), "cps_hip_matpc_dag_matpc_prec")
+        return out, dp
+
+    def matpc(self, op, inp, dag=0, nrhs=1):
+        out = np.zeros(self._nreal(op) * nrhs)
+        _check(lib().cps_hip_matpc(self.h, ctypes.byref(op), _ptr(out), _ptr(inp), dag, nrhs), "cps_hip_matpc")
+        return out
+
+    def clover_build(self, kappas, csw):
+        k = np.ascontiguousarray(np.asarray(kappas, dtype=np.float64))
+        _check(lib().cps_hip_clover_build(self.h, _ptr(k), len(k), csw), "cps_hip_clover_build")
+
+    def clover_export(self, k, cb):
+        A = np.zeros(72 * self.vcb)
+        _check(lib().cps_hip_clover_export(self.h, k, cb, _ptr(A)), "cps_hip_clover_export")
+        return A
+
+    def invcg(self, op, src, arg, guess=None, nrhs=1):
+        sol = np.zeros(self._nreal(op) * nrhs) if guess is None else np.ascontiguousarray(guess, dtype=np.float64).copy()
+        res = (CgResult * nrhs)()
+        rc = lib().cps_hip_invcg(self.h, ctypes.byref(op), _ptr(sol), _ptr(src), nrhs, ctypes.byref(arg), res)
+        _check(rc, "cps_hip_invcg")
+        return sol, [res[k] for k in range(nrhs)]
+
+    def matinv(self, op, b, arg, guess=None, nrhs=1):
+        out = np.zeros(2 * self._nreal(op) * nrhs) if guess is None else np.ascontiguousarray(guess, dtype=np.float64).copy()
+        res = (CgResult * nrhs)()
+        _check(lib().cps_hip_matinv(self.h, ctypes.byref(op), _ptr(out), _ptr(b), nrhs, ctypes.byref(arg), res),
+               "cps_hip_matinv")
+        return out, [res[k] for k in range(nrhs)]
+
+    def bench_hop(self, kind, precision, nrhs=1, reps=20):
+        s, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().cps_hip_bench_hop(self.h, kind, precision, nrhs, reps, ctypes.byref(s), ctypes.byref(b)),
+               "cps_hip_bench_hop")
+        return s.value, b.value
+
+
+def device_count():
+    return int(lib().cps_hip_device_count())

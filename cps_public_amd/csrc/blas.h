@@ -1,0 +1,46 @@
+// Vector BLAS and CG scalar kernels (see blas.hip).
+#pragma once
+#include "cps_hip_internal.h"
+#include "layout.h"
+
+namespace cpship {
+
+enum ReduceMode { RED_PLAIN = 0, RED_CG_ALPHA = 1, RED_CG_BETA = 2, RED_CG_INIT = 3, RED_SRC_NORM = 4 };
+enum CGStatus {
+  CG_STATUS_RUNNING = 0, CG_STATUS_CONVERGED = 1, CG_STATUS_MAXITER = 2, CG_STATUS_D_ZERO = 3,
+  CG_STATUS_NEED_UPDATE = 4   // mixed precision: sloppy residual fell below delta * last true residual
+};
+
+// per-rhs CG scalars, all device arrays of length nrhs
+struct CGScalars {
+  double *rr, *rr_prev, *d, *alpha, *beta, *stp, *src_norm, *rsd;
+  int *active, *iters, *status;
+  int max_itr;        // max_num_iter - 1 (inv_cg.C:275)
+  double* rr_upd;     // mixed: true |r|^2 at the last reliable update (nullptr for FP64 CG)
+  double delta2;      // mixed: reliable-update trigger |r_s|^2 < delta2 * rr_upd
+  double* hist;       // optional |r|^2 history ring [nrhs][hist_len]
+  int hist_len;
+};
+
+hipError_t blas_norm(int st, const FieldRef& x, int64_t per_rhs, int nrhs, int bs, double* partial, hipStream_t s);
+hipError_t blas_redot(int st, const FieldRef& x, const FieldRef& y, int64_t per_rhs, int nrhs, int bs,
+                      double* partial, hipStream_t s);
+hipError_t blas_sub_norm_d(const FieldRef& b, const FieldRef& a, const FieldRef& r, int64_t per_rhs, int nrhs, int bs,
+                           double* partial, hipStream_t s);
+hipError_t blas_cg_xr_d(const FieldRef& x, const FieldRef& p, const FieldRef& r, const FieldRef& ap, const CGScalars& cg,
+                        int64_t per_rhs, int nrhs, int bs, double* partial, hipStream_t s);
+hipError_t blas_xpay(int st, const FieldRef& p, const FieldRef& r, const double* beta, const int* active,
+                     int64_t per_rhs, int nrhs, int bs, hipStream_t s);
+hipError_t blas_axpy(int sx, const FieldRef& x, int sy, const FieldRef& y, const double* a, double a_const,
+                     int64_t per_rhs, int nrhs, int bs, hipStream_t s);
+hipError_t reduce_rhs(const double* partial, int bpr, int nrhs, double* out, int mode, const CGScalars& cg,
+                      hipStream_t s);
+
+hipError_t blas_cg_xr_mixed(int sx, const FieldRef& x, int sp, const FieldRef& p, int sr, const FieldRef& r, int sa,
+                            const FieldRef& ap, const CGScalars& cg, int64_t per_rhs, int nrhs, int bs,
+                            double* partial, hipStream_t s);
+hipError_t blas_xpay_mixed(int sp, const FieldRef& p, int sr, const FieldRef& r, const double* beta, const int* active,
+                           int64_t per_rhs, int nrhs, int bs, hipStream_t s);
+hipError_t blas_zero(int st, const FieldRef& x, int64_t per_rhs, int nrhs, int bs, hipStream_t s);
+
+}  // namespace cpship

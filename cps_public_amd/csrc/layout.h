@@ -1,0 +1,31 @@
+// Layout transfer and precision conversion (see layout.hip).
+#pragma once
+#include "cps_hip_internal.h"
+
+namespace cpship {
+
+enum SpinStorage { STORE_D = 0, STORE_F = 1, STORE_H = 2 };
+enum LinkStorage { LINK_D18 = 0, LINK_F12 = 1, LINK_H12 = 2, LINK_B12 = 3 };
+enum SpinMap { MAP_4D = 0, MAP_DWF5 = 1, MAP_4DPREC = 2 };
+
+hipError_t pack_spinor(int storage, const double* aos, const FieldRef& out, int map, int Vcb, int ls, int cb5,
+                       int64_t n, hipStream_t st);
+hipError_t unpack_spinor(int storage, const FieldRef& in, double* aos, int map, int Vcb, int ls, int cb5,
+                         int64_t n, hipStream_t st);
+hipError_t convert_spinor(int sin, const FieldRef& in, int sout, const FieldRef& out, int64_t n, hipStream_t st);
+hipError_t pack_gauge(int link_storage, const double* aos, const GaugeRef& out, int Vcb, hipStream_t st);
+
+inline size_t spin_elem_bytes(int storage) {
+  return storage == STORE_D ? 16 : storage == STORE_F ? 8 : 4;
+}
+inline size_t link_elem_bytes(int ls_) {
+  return ls_ == LINK_D18 ? 16 : ls_ == LINK_F12 ? 8 : 4;
+}
+inline int link_planes(int ls_) { return ls_ == LINK_D18 ? 9 : 6; }
+
+}  // namespace cpship
+
+namespace cpship {
+// deterministic pseudo-random fill in [-1,1) (benchmark inputs)
+hipError_t fill_spinor(int storage, const FieldRef& f, int64_t n, unsigned seed, hipStream_t st);
+}

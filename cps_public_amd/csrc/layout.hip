@@ -1,0 +1,200 @@
+// Layout transfer between the CPS WILSON-order arrays (array of structures,
+// FP64) and the device structure-of-arrays fields, plus precision
+// conversion between device fields.
+//
+// CPS orders followed (convert_func.C:318-494, convert.C:635-760):
+//   4-D fermion checkerboard : [site_cb][spin][colour][re,im]
+//   5-D DWF checkerboard cb  : slices s = 0..Ls-1 of Vcb sites, slice s at
+//                              4-D parity (s+cb)%2
+//   4-D-preconditioned 5-D   : slices s = 0..Ls-1 of Vcb sites of one parity
+//   gauge                    : [parity][site_cb][mu][col][row][re,im]
+#include "cps_hip_internal.h"
+#include "layout.h"
+
+namespace cpship {
+
+__device__ __forceinline__ int64_t soa_index(int map, int64_t k, int Vcb, int ls, int cb5, int nrhs_unused) {
+  // k enumerates CPS order: rhs-major, then (s, site_cb) or site_cb
+  (void)nrhs_unused;
+  if (map == MAP_4D) return k;
+  const int64_t per = (int64_t)Vcb * ls;
+  const int64_t rhs = k / per;
+  const int64_t rem = k - rhs * per;
+  const int s = (int)(rem / Vcb);
+  const int i = (int)(rem - (int64_t)s * Vcb);
+  if (map == MAP_DWF5) {
+    const int p = (s + cb5) & 1;
+    const int nb = ls >> 1;
+    return ((rhs * 2 + p) * Vcb + i) * nb + (s >> 1);
+  }
+  // MAP_4DPREC
+  return (rhs * Vcb + i) * ls + s;
+}
+
+template <typename S>
+__global__ void k_pack_spinor(const double* __restrict__ aos, FieldRef out, int map, int Vcb, int ls, int cb5,
+                              int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  using R = typename SpinTraits<S>::R;
+  cplx<R> v[12];
+  const double* p = aos + 24 * k;
+#pragma unroll
+  for (int c = 0; c < 12; c++) v[c] = cmk<R>(R(p[2 * c]), R(p[2 * c + 1]));
+  spin_store<S, R>(out, soa_index(map, k, Vcb, ls, cb5, 0), v);
+}
+
+template <typename S>
+__global__ void k_unpack_spinor(FieldRef in, double* __restrict__ aos, int map, int Vcb, int ls, int cb5,
+                                int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  using R = typename SpinTraits<S>::R;
+  cplx<R> v[12];
+  spin_load<S, R>(in, soa_index(map, k, Vcb, ls, cb5, 0), v);
+  double* p = aos + 24 * k;
+#pragma unroll
+  for (int c = 0; c < 12; c++) {
+    p[2 * c] = double(v[c].re);
+    p[2 * c + 1] = double(v[c].im);
+  }
+}
+
+template <typename SIN, typename SOUT>
+__global__ void k_convert(FieldRef in, FieldRef out, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  using R = float;
+  if constexpr (std::is_same<SIN, SpinD>::value && std::is_same<SOUT, SpinD>::value) {
+    cplx<double> v[12];
+    spin_load<SIN, double>(in, e, v);
+    spin_store<SOUT, double>(out, e, v);
+  } else {
+    using RI = typename SpinTraits<SIN>::R;
+    cplx<RI> v[12];
+    spin_load<SIN, RI>(in, e, v);
+    using RO = typename SpinTraits<SOUT>::R;
+    cplx<RO> w[12];
+#pragma unroll
+    for (int c = 0; c < 12; c++) w[c] = cmk<RO>(RO(v[c].re), RO(v[c].im));
+    spin_store<SOUT, RO>(out, e, w);
+  }
+}
+
+template <typename E> __device__ __forceinline__ E make_link_elem(double re, double im);
+template <> __device__ __forceinline__ double2 make_link_elem<double2>(double re, double im) { return make_double2(re, im); }
+template <> __device__ __forceinline__ float2 make_link_elem<float2>(double re, double im) { return make_float2(float(re), float(im)); }
+template <> __device__ __forceinline__ __half2 make_link_elem<__half2>(double re, double im) { return __floats2half2_rn(float(re), float(im)); }
+template <> __device__ __forceinline__ __hip_bfloat162 make_link_elem<__hip_bfloat162>(double re, double im) {
+  __hip_bfloat162 b;
+  b.x = __float2bfloat16(float(re));
+  b.y = __float2bfloat16(float(im));
+  return b;
+}
+
+// one thread per (parity, site_cb, mu)
+template <typename LK>
+__global__ void k_pack_gauge(const double* __restrict__ aos, GaugeRef out, int Vcb) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)8 * Vcb;
+  if (k >= n) return;
+  const int mu = (int)(k & 3);
+  const int64_t ps = k >> 2;                 // parity*Vcb + site
+  const int parity = (int)(ps / Vcb);
+  const int site = (int)(ps - (int64_t)parity * Vcb);
+  const double* src = aos + 72 * ps + 18 * mu;
+  using E = typename LinkTraits<LK>::elem;
+  E* dst = reinterpret_cast<E*>(const_cast<void*>(out.data)) + (int64_t)(parity * 4 + mu) * Vcb + site;
+  const int np = LinkTraits<LK>::nplanes;
+  for (int row = 0; row < np / 3; row++)
+    for (int col = 0; col < 3; col++) {
+      const int w = 2 * (row + 3 * col);   // WILSON order: U(row,col) at 2*(row+3*col)
+      dst[(3 * row + col) * out.plane] = make_link_elem<E>(src[w], src[w + 1]);
+    }
+}
+
+static inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t pack_spinor(int storage, const double* aos, const FieldRef& out, int map, int Vcb, int ls, int cb5,
+                       int64_t n, hipStream_t st) {
+  const int bs = 256;
+  switch (storage) {
+    case STORE_D: hipLaunchKernelGGL(k_pack_spinor<SpinD>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, map, Vcb, ls, cb5, n); break;
+    case STORE_F: hipLaunchKernelGGL(k_pack_spinor<SpinF>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, map, Vcb, ls, cb5, n); break;
+    case STORE_H: hipLaunchKernelGGL(k_pack_spinor<SpinH>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, map, Vcb, ls, cb5, n); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t unpack_spinor(int storage, const FieldRef& in, double* aos, int map, int Vcb, int ls, int cb5,
+                         int64_t n, hipStream_t st) {
+  const int bs = 256;
+  switch (storage) {
+    case STORE_D: hipLaunchKernelGGL(k_unpack_spinor<SpinD>, dim3(nblocks(n, bs)), dim3(bs), 0, st, in, aos, map, Vcb, ls, cb5, n); break;
+    case STORE_F: hipLaunchKernelGGL(k_unpack_spinor<SpinF>, dim3(nblocks(n, bs)), dim3(bs), 0, st, in, aos, map, Vcb, ls, cb5, n); break;
+    case STORE_H: hipLaunchKernelGGL(k_unpack_spinor<SpinH>, dim3(nblocks(n, bs)), dim3(bs), 0, st, in, aos, map, Vcb, ls, cb5, n); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t convert_spinor(int sin, const FieldRef& in, int sout, const FieldRef& out, int64_t n, hipStream_t st) {
+  const int bs = 256;
+  dim3 g(nblocks(n, bs)), b(bs);
+#define CPSHIP_CONV(A, TA, B, TB) \
+  if (sin == A && sout == B) { hipLaunchKernelGGL((k_convert<TA, TB>), g, b, 0, st, in, out, n); return hipGetLastError(); }
+  CPSHIP_CONV(STORE_D, SpinD, STORE_D, SpinD)
+  CPSHIP_CONV(STORE_D, SpinD, STORE_F, SpinF)
+  CPSHIP_CONV(STORE_D, SpinD, STORE_H, SpinH)
+  CPSHIP_CONV(STORE_F, SpinF, STORE_D, SpinD)
+  CPSHIP_CONV(STORE_F, SpinF, STORE_F, SpinF)
+  CPSHIP_CONV(STORE_F, SpinF, STORE_H, SpinH)
+  CPSHIP_CONV(STORE_H, SpinH, STORE_D, SpinD)
+  CPSHIP_CONV(STORE_H, SpinH, STORE_F, SpinF)
+  CPSHIP_CONV(STORE_H, SpinH, STORE_H, SpinH)
+#undef CPSHIP_CONV
+  return hipErrorInvalidValue;
+}
+
+hipError_t pack_gauge(int link_storage, const double* aos, const GaugeRef& out, int Vcb, hipStream_t st) {
+  const int bs = 256;
+  const int64_t n = (int64_t)8 * Vcb;
+  switch (link_storage) {
+    case LINK_D18: hipLaunchKernelGGL(k_pack_gauge<LinkD18>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, Vcb); break;
+    case LINK_F12: hipLaunchKernelGGL(k_pack_gauge<LinkF12>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, Vcb); break;
+    case LINK_H12: hipLaunchKernelGGL(k_pack_gauge<LinkH12>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, Vcb); break;
+    case LINK_B12: hipLaunchKernelGGL(k_pack_gauge<LinkB12>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, Vcb); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+__device__ __forceinline__ float hash_unit(uint64_t k) {
+  k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
+  return float((k >> 40) & 0xFFFFFF) * (2.0f / 16777216.0f) - 1.0f;
+}
+
+template <typename S>
+__global__ void k_fill(FieldRef f, int64_t n, unsigned seed) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  using R = typename SpinTraits<S>::R;
+  cplx<R> v[12];
+#pragma unroll
+  for (int c = 0; c < 12; c++) {
+    const uint64_t k = ((uint64_t)seed << 40) ^ ((uint64_t)e * 24 + 2 * c);
+    v[c] = cmk<R>(R(hash_unit(k)), R(hash_unit(k + 1)));
+  }
+  spin_store<S, R>(f, e, v);
+}
+
+hipError_t fill_spinor(int storage, const FieldRef& f, int64_t n, unsigned seed, hipStream_t st) {
+  const int bs = 256;
+  if (storage == STORE_D) hipLaunchKernelGGL(k_fill<SpinD>, dim3(nblocks(n, bs)), dim3(bs), 0, st, f, n, seed);
+  else if (storage == STORE_F) hipLaunchKernelGGL(k_fill<SpinF>, dim3(nblocks(n, bs)), dim3(bs), 0, st, f, n, seed);
+  else hipLaunchKernelGGL(k_fill<SpinH>, dim3(nblocks(n, bs)), dim3(bs), 0, st, f, n, seed);
+  return hipGetLastError();
+}
+
+}  // namespace cpship

@@ -1,0 +1,359 @@
+// Moebius domain wall (4-D even-odd preconditioned, constant real b and c).
+//
+// Follows zmobius_m_orig / zmobius_mdag_orig (zmobius_m-orig.h:3-122,
+// zmobius_mdag-orig.h) of the CPS noarch Moebius operator:
+//   M   = 1 + (kb/kc) D5 - kb D4(b,c) M5^-1 kb D4(b,c)
+// with D4(b,c) = Wilson hop of (b + c P(+-) shift) and M5^-1 the LDU
+// recurrence of zmobius_m5inv.C:247-686.  Fields are single 4-D parity with
+// the Ls slices as the inner batch index, so the 4-D hop is the Wilson
+// kernel with nb = Ls (links loaded once per site for all s).
+//
+// Kernels here:
+//   k_s5     out = xc*X + a1*(b1*A + c1*P A) + a2*(b2*B + c2*P B)
+//            P the CPS fifth-dimension shift (dwf_dslash_5_plus / the
+//            zmobius kappa_dslash_5_plus) with -mass at the walls
+//   k_m5inv  in-place M5^-1; a block stages S sites x Ls slices in LDS, one
+//            thread per (site, component) runs the recurrence over s
+#include "solver.h"
+
+#include <cstring>
+
+namespace cpship {
+
+struct S5Params {
+  FieldRef out, A, B, X;
+  double a1, b1, c1, a2, b2, c2, xc;
+  double mass;
+  int dag, ls, nrhs;
+  int64_t per_rhs;  // Vcb*Ls
+  double* partial;
+};
+
+// value of the CPS shift term (P+ from s-1, P- from s+1 for dag 0; exchanged
+// for dag 1) at slot s of a site whose Ls slots start at element `base`
+template <typename S, typename R>
+__device__ __forceinline__ void s5_term(const FieldRef& f, int64_t base, int s, int ls, int dag, R mass,
+                                        cplx<R> t[12]) {
+  const int sm = (s == 0) ? ls - 1 : s - 1;
+  const int sp = (s == ls - 1) ? 0 : s + 1;
+  const R cm = (s == 0) ? -mass : R(1);
+  const R cp = (s == ls - 1) ? -mass : R(1);
+  const int c_from_m = dag ? 6 : 0;
+  const int c_from_p = dag ? 0 : 6;
+  cplx<R> hm[6], hp[6];
+  spin_load_half<S, R>(f, base + sm, c_from_m, hm);
+  spin_load_half<S, R>(f, base + sp, c_from_p, hp);
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    t[c_from_m + k] = cmk<R>(cm * hm[k].re, cm * hm[k].im);
+    t[c_from_p + k] = cmk<R>(cp * hp[k].re, cp * hp[k].im);
+  }
+}
+
+template <typename S>
+__global__ void k_s5(S5Params P) {
+  using R = typename SpinTraits<S>::R;
+  const int rhs = blockIdx.y;
+  const int64_t e = (int64_t)rhs * P.per_rhs + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = (int)(e % P.ls);
+  const int64_t base = e - s;
+  const R mass = R(P.mass);
+  cplx<R> res[12];
+  {
+    cplx<R> v[12], t[12];
+    spin_load<S, R>(P.A, e, v);
+    s5_term<S, R>(P.A, base, s, P.ls, P.dag, mass, t);
+    const R a = R(P.a1), b = R(P.b1), c = R(P.c1);
+#pragma unroll
+    for (int k = 0; k < 12; k++) res[k] = cmk<R>(a * (b * v[k].re + c * t[k].re), a * (b * v[k].im + c * t[k].im));
+  }
+  if (P.B.data) {
+    cplx<R> v[12], t[12];
+    spin_load<S, R>(P.B, e, v);
+    s5_term<S, R>(P.B, base, s, P.ls, P.dag, mass, t);
+    const R a = R(P.a2), b = R(P.b2), c = R(P.c2);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      res[k].re += a * (b * v[k].re + c * t[k].re);
+      res[k].im += a * (b * v[k].im + c * t[k].im);
+    }
+  }
+  if (P.X.data) {
+    cplx<R> v[12];
+    spin_load<S, R>(P.X, e, v);
+    const R xc = R(P.xc);
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      res[k].re += xc * v[k].re;
+      res[k].im += xc * v[k].im;
+    }
+  }
+  spin_store<S, R>(P.out, e, res);
+  if (P.partial) {
+    double n = 0;
+#pragma unroll
+    for (int k = 0; k < 12; k++) n += double(res[k].re) * res[k].re + double(res[k].im) * res[k].im;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    __shared__ double red[16];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0;
+      for (int w = 0; w < (int)((blockDim.x + 63) >> 6); w++) t += red[w];
+      P.partial[(int64_t)rhs * gridDim.x + blockIdx.x] = t;
+    }
+  }
+}
+
+struct M5Params {
+  FieldRef f;
+  int ls, dag;
+  double K, mass;
+  int64_t nsites;   // nrhs * Vcb
+};
+
+// zmobius_m5inv_cmplx_dag0 / _dag1 with constant real K, one component
+template <typename R>
+__device__ __forceinline__ void m5inv_component(R* v /* [s] stride 12 */, int stride, int ls, int dag, int lower,
+                                                R K, R mass) {
+#define V(s_) v[(s_) * stride]
+  if (dag == 0) {
+    R prod = 1;
+    for (int i = 0; i < ls; i++) prod *= -K;
+    prod *= mass;
+    const R inv_d_last = R(1) / (R(1) + prod);
+    if (!lower) {
+      for (int s = 0; s <= ls - 2; ++s) V(s + 1) += -K * V(s);
+      R prd = 1;
+      for (int i = 0; i < ls - 1; i++) prd *= -K;
+      R factR = -prd * mass * inv_d_last;
+      for (int s = ls - 2; s >= 0; --s) {
+        V(s) += factR * V(ls - 1);
+        factR /= -K;
+      }
+      V(ls - 1) *= inv_d_last;
+    } else {
+      V(ls - 1) *= inv_d_last;
+      R factL = K * mass * inv_d_last;
+      for (int s = 0; s <= ls - 2; ++s) {
+        V(ls - 1) += factL * V(s);
+        factL *= -K;
+      }
+      for (int s = ls - 2; s >= 0; --s) V(s) += -K * V(s + 1);
+    }
+  } else {
+    R prod = 1;
+    for (int i = 0; i < ls; i++) prod *= K;
+    prod *= mass;
+    const R inv_d_last = R(1) / (R(1) + prod);
+    if (!lower) {
+      V(ls - 1) *= inv_d_last;
+      R factR = K * mass * inv_d_last;
+      for (int s = 0; s <= ls - 2; ++s) {
+        V(ls - 1) += factR * V(s);
+        factR *= -K;
+      }
+      for (int s = ls - 2; s >= 0; --s) V(s) += -K * V(s + 1);
+    } else {
+      for (int s = 0; s <= ls - 2; ++s) V(s + 1) += -K * V(s);
+      R prd = 1;
+      for (int i = 0; i < ls - 2; i++) prd *= -K;
+      prd *= -K;
+      R factL = -prd * mass * inv_d_last;
+      for (int s = ls - 2; s >= 0; --s) {
+        V(s) += factL * V(ls - 1);
+        if (s > 0) factL /= -K;
+      }
+      V(ls - 1) *= inv_d_last;
+    }
+  }
+#undef V
+}
+
+// block: SITES sites x ls slots; LDS holds the block's fields in compute precision
+template <typename S>
+__global__ void k_m5inv(M5Params P) {
+  using R = typename SpinTraits<S>::R;
+  extern __shared__ unsigned char smem_raw[];
+  R* lds = reinterpret_cast<R*>(smem_raw);   // [site_local][s][24]
+  const int ls = P.ls;
+  const int sites_per_block = blockDim.x / ls;
+  const int64_t site0 = (int64_t)blockIdx.x * sites_per_block;
+  // phase 1: element (site_local, s) -> LDS
+  {
+    const int sl = threadIdx.x / ls, s = threadIdx.x - (threadIdx.x / ls) * ls;
+    const int64_t site = site0 + sl;
+    if (site < P.nsites) {
+      cplx<R> v[12];
+      spin_load<S, R>(P.f, site * ls + s, v);
+      R* d = lds + ((int64_t)sl * ls + s) * 24;
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+        d[2 * c] = v[c].re;
+        d[2 * c + 1] = v[c].im;
+      }
+    }
+  }
+  __syncthreads();
+  // phase 2: (site_local, real component r in 0..23) runs the recurrence
+  for (int w = threadIdx.x; w < sites_per_block * 24; w += blockDim.x) {
+    const int sl = w / 24, r = w - sl * 24;
+    if (site0 + sl < P.nsites) {
+      const int lower = r >= 12;   // spins 2,3
+      m5inv_component<R>(lds + (int64_t)sl * ls * 24 + r, 24, ls, P.dag, lower, R(P.K), R(P.mass));
+    }
+  }
+  __syncthreads();
+  {
+    const int sl = threadIdx.x / ls, s = threadIdx.x - (threadIdx.x / ls) * ls;
+    const int64_t site = site0 + sl;
+    if (site < P.nsites) {
+      cplx<R> v[12];
+      const R* d = lds + ((int64_t)sl * ls + s) * 24;
+#pragma unroll
+      for (int c = 0; c < 12; c++) v[c] = cmk<R>(d[2 * c], d[2 * c + 1]);
+      spin_store<S, R>(P.f, site * ls + s, v);
+    }
+  }
+}
+
+static int store_of(int hprec) { return hprec == PREC_D_D18 ? STORE_D : hprec == PREC_F_F12 ? STORE_F : STORE_H; }
+
+static int launch_s5(Lattice& L, int hprec, S5Params& P) {
+  const int bs = pick_block(P.per_rhs);
+  dim3 grid((unsigned)(P.per_rhs / bs), (unsigned)P.nrhs), block(bs);
+  switch (store_of(hprec)) {
+    case STORE_D: hipLaunchKernelGGL(k_s5<SpinD>, grid, block, 0, L.stream, P); break;
+    case STORE_F: hipLaunchKernelGGL(k_s5<SpinF>, grid, block, 0, L.stream, P); break;
+    default: hipLaunchKernelGGL(k_s5<SpinH>, grid, block, 0, L.stream, P); break;
+  }
+  CPSHIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+static int launch_m5inv(Lattice& L, int hprec, const FieldRef& f, int dag, double K, double mass, int nrhs) {
+  M5Params P;
+  P.f = f;
+  P.ls = L.ls;
+  P.dag = dag;
+  P.K = K;
+  P.mass = mass;
+  P.nsites = (int64_t)nrhs * L.g.Vcb;
+  const int sites_per_block = std::max(1, 256 / L.ls);
+  const int bs = sites_per_block * L.ls;
+  const int st = store_of(hprec);
+  const size_t rbytes = st == STORE_D ? sizeof(double) : sizeof(float);
+  const size_t lds = (size_t)sites_per_block * L.ls * 24 * rbytes;
+  dim3 grid((unsigned)((P.nsites + sites_per_block - 1) / sites_per_block)), block(bs);
+  switch (st) {
+    case STORE_D: hipLaunchKernelGGL(k_m5inv<SpinD>, grid, block, lds, L.stream, P); break;
+    case STORE_F: hipLaunchKernelGGL(k_m5inv<SpinF>, grid, block, lds, L.stream, P); break;
+    default: hipLaunchKernelGGL(k_m5inv<SpinH>, grid, block, lds, L.stream, P); break;
+  }
+  CPSHIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+static void mobius_kappas(const cps_hip_op* op, double* kb, double* kc) {
+  // d_op_mobius.C:106-109
+  *kb = 1.0 / (2 * (op->mobius_b * (4 - op->m5) + op->a5_inv));
+  *kc = 1.0 / (2 * (op->mobius_c * (4 - op->m5) - op->a5_inv));
+}
+
+// 4-D hop of every slice (Wilson kernel, nb = Ls), output parity `target`
+static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag, double scale,
+                int nrhs) {
+  cps_hip_op w;
+  std::memset(&w, 0, sizeof(w));
+  w.kind = CPS_HIP_OP_MOBIUS;   // single-parity layout with nb = Ls
+  HopParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.geom = L.g;
+  P.kind = KIND_WILSON;
+  P.prec = hprec;
+  P.dag = dag;
+  P.nb = L.ls;
+  P.nrhs = nrhs;
+  P.nsites_out = L.g.Vcb;
+  P.out_parity = target;
+  P.block = pick_block((long long)L.g.Vcb * L.ls);
+  P.link_bc_sign = (hprec != PREC_D_D18);
+  P.scale = scale;
+  P.in = in;
+  P.out = out;
+  P.X = null_field();
+  const int lstore = hprec == PREC_D_D18 ? LINK_D18 : hprec == PREC_F_F12 ? LINK_F12 : hprec == PREC_H_H12 ? LINK_H12 : LINK_B12;
+  int rc = L.ensure_gauge(lstore);
+  if (rc) return rc;
+  P.gauge = L.gauge_ref(lstore);
+  CPSHIP_CHECK(launch_hop(P, L.stream));
+  return 0;
+}
+
+static S5Params s5_base(Lattice& L, const cps_hip_op* op, int dag, int nrhs) {
+  S5Params P;
+  std::memset(&P, 0, sizeof(P));
+  P.mass = op->mass;
+  P.dag = dag;
+  P.ls = L.ls;
+  P.nrhs = nrhs;
+  P.per_rhs = (int64_t)L.g.Vcb * L.ls;
+  P.A = P.B = P.X = null_field();
+  return P;
+}
+
+// zmobius_m_orig (dag 0) / zmobius_mdag_orig (dag 1); t1, t2 scratch
+int mobius_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+             const FieldRef& t1, const FieldRef& t2, int dag, double* partial, int nrhs) {
+  double kb, kc;
+  mobius_kappas(op, &kb, &kc);
+  const double ratio = kb / kc;
+  const double b = op->mobius_b, c = op->mobius_c;
+  int rc;
+  if (dag == 0) {
+    // t1 = b in + c P in ; out = kb * D4(cb=1) t1 ; M5^-1 out
+    S5Params P = s5_base(L, op, 0, nrhs);
+    P.out = t1; P.A = in; P.a1 = 1; P.b1 = b; P.c1 = c;
+    if ((rc = launch_s5(L, hprec, P))) return rc;
+    if ((rc = hop4(L, hprec, out, t1, 0, 0, kb, nrhs))) return rc;
+    if ((rc = launch_m5inv(L, hprec, out, 0, ratio, op->mass, nrhs))) return rc;
+    // t1 = b out + c P out ; t2 = -kb * D4(cb=0) t1
+    P = s5_base(L, op, 0, nrhs);
+    P.out = t1; P.A = out; P.a1 = 1; P.b1 = b; P.c1 = c;
+    if ((rc = launch_s5(L, hprec, P))) return rc;
+    if ((rc = hop4(L, hprec, t2, t1, 1, 0, -kb, nrhs))) return rc;
+    // out = in + t2 + (kb/kc) P in
+    P = s5_base(L, op, 0, nrhs);
+    P.out = out; P.A = in; P.a1 = 1; P.b1 = 1; P.c1 = ratio; P.X = t2; P.xc = 1;
+    P.partial = partial;
+    return launch_s5(L, hprec, P);
+  }
+  // dag 1: t1 = -kb * D4dag(cb=1) in ; t2 = b t1 + c Pdag t1 ; M5^-dag t2
+  if ((rc = hop4(L, hprec, t1, in, 0, 1, -kb, nrhs))) return rc;
+  S5Params P = s5_base(L, op, 1, nrhs);
+  P.out = t2; P.A = t1; P.a1 = 1; P.b1 = b; P.c1 = c;
+  if ((rc = launch_s5(L, hprec, P))) return rc;
+  if ((rc = launch_m5inv(L, hprec, t2, 1, ratio, op->mass, nrhs))) return rc;
+  // t1 = kb * D4dag(cb=0) t2 ; out = (b t1 + c Pdag t1) + (in + ratio Pdag in)
+  if ((rc = hop4(L, hprec, t1, t2, 1, 1, kb, nrhs))) return rc;
+  P = s5_base(L, op, 1, nrhs);
+  P.out = out; P.A = t1; P.a1 = 1; P.b1 = b; P.c1 = c;
+  P.B = in; P.a2 = 1; P.b2 = 1; P.c2 = ratio;
+  P.partial = partial;
+  return launch_s5(L, hprec, P);
+}
+
+// mobius_mdagm (mobius_mdagm.C:33-60): t3 = M in, |t3|^2, out = M^dag t3.
+// t1, t2 are the operator scratch; the intermediate uses scratch slot S_T3/S_T3S.
+int mobius_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                 const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs) {
+  const int st = store_of(hprec);
+  Field& t3 = L.field(st == STORE_D ? S_T3 : S_T3S, st, (int64_t)nrhs * L.g.Vcb * L.ls);
+  if (L.last_rc) return L.last_rc;
+  int rc = mobius_m(L, op, hprec, t3.ref(), in, t1, t2, 0, partial, nrhs);
+  if (rc) return rc;
+  return mobius_m(L, op, hprec, out, t3.ref(), t1, t2, 1, nullptr, nrhs);
+}
+
+}  // namespace cpship

@@ -1,0 +1,121 @@
+// Host-side objects behind the C ABI.
+#pragma once
+#include <vector>
+
+#include "../../include/cps_hip.h"
+#include "blas.h"
+#include "cps_hip_internal.h"
+#include "hop.h"
+#include "layout.h"
+
+namespace cpship {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  ~DevBuf() { release(); }
+  int ensure(size_t b);
+  void release();
+};
+
+struct Field {
+  int storage = -1;
+  int64_t nelem = 0;
+  DevBuf buf, nrm;
+  int alloc(int st, int64_t n);
+  FieldRef ref() const;
+};
+
+FieldRef null_field();
+
+enum Slot {
+  S_X = 0, S_SRC, S_R, S_P, S_AP, S_T1, S_T2, S_XS, S_RS, S_PS, S_APS, S_T1S, S_T2S, S_T3, S_T3S,
+  S_IO_IN, S_IO_OUT, S_EVEN_IN, S_EVEN_OUT, S_TMP, S_BENCH_IN, S_BENCH_OUT, S_NSLOTS
+};
+
+struct OpLayout {
+  int kind, map, nb, nsites;
+  int64_t per_rhs;   // elements of one rhs
+};
+
+struct Lattice {
+  Geom g;
+  int ls = 1;
+  hipStream_t stream = nullptr;
+  DevBuf gauge_aos;
+  DevBuf gauge[4];
+  bool gauge_valid[4];
+  bool has_gauge = false;
+  long long gauge_generation = 0;
+  std::vector<Field> scratch;
+  DevBuf partial, scal, io, io2;
+  // clover: per kappa, A_oo and A_ee^-1 packed (72 doubles per site) for
+  // each checkerboard, layout [k][cb][site][72]
+  DevBuf clover;
+  int n_clover = 0;
+  std::vector<double> clover_kappas;
+  double clover_csw = 0;
+  int last_rc = 0;
+
+  Lattice();
+  ~Lattice();
+  int init(const int node_sites[4], int ls, const int bc[4]);
+  int import_gauge(const double* u_host);
+  int ensure_gauge(int link_storage);
+  GaugeRef gauge_ref(int link_storage) const;
+  Field& field(int slot, int storage, int64_t nelem);
+  int ensure_partial(int64_t slots);
+  long long device_bytes() const;
+};
+
+struct CGState {
+  int nrhs = 0;
+  CGScalars s;
+  double* rr_true = nullptr;    // FP64 |r|^2 per rhs (true residual evaluations)
+  double* upd_store = nullptr;  // backing store of s.rr_upd
+  int* mask = nullptr;          // per-rhs flags for masked BLAS launches
+  int alloc(Lattice& L, int nrhs, int hist_len);
+};
+
+int op_layout(const cps_hip_op* op, int ls, OpLayout* o, const Geom& g);
+double op_kappa(const cps_hip_op* op);
+int hop(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
+        double scale, const FieldRef& X, double xcoef, double* partial, int nrhs);
+int hop_blocks_per_rhs(Lattice& L, const cps_hip_op* op);
+int apply_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, const FieldRef& t,
+            int dag, double* partial, int nrhs);
+int apply_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs);
+int apply_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                const FieldRef& t1, const FieldRef& t2, int dag, int nrhs);
+
+// Moebius (mobius.hip) and clover (clover.hip)
+int mobius_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+             const FieldRef& t1, const FieldRef& t2, int dag, double* partial, int nrhs);
+int mobius_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                 const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs);
+int clover_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                 const FieldRef& t1, int dag, double* partial, int nrhs);
+int clover_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                 const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs);
+int clover_build(Lattice& L, const double* kappas, int n_kappa, double csw);
+
+int upload(Lattice& L, const double* host, Field& f, const OpLayout& o, int cb5, int nrhs, int slot_io);
+int download(Lattice& L, const Field& f, double* host, const OpLayout& o, int cb5, int nrhs);
+
+int cg_double_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int nrhs, const cps_hip_cg_arg* arg,
+                     std::vector<int>& iters, std::vector<int>& status, std::vector<double>& true_res,
+                     std::vector<double>& src_norm);
+int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int nrhs, const cps_hip_cg_arg* arg,
+                    std::vector<int>& iters, std::vector<int>& status, std::vector<double>& true_res,
+                    std::vector<double>& src_norm, std::vector<int>& n_updates);
+
+}  // namespace cpship
+
+struct cps_hip_lattice {
+  cpship::Lattice L;
+};

@@ -1,0 +1,631 @@
+// Host driver: lattice handle, device fields, operator application and the
+// batched conjugate-gradient solvers behind the C ABI in include/cps_hip.h.
+#include "solver.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace cpship {
+
+// ---------------------------------------------------------------------------
+int DevBuf::ensure(size_t b) {
+  if (b <= bytes) return 0;
+  release();
+  if (b == 0) return 0;
+  if (hipMalloc(&p, b) != hipSuccess) {
+    p = nullptr;
+    bytes = 0;
+    (void)hipGetLastError();
+    return CPS_HIP_ERR_NOMEM;
+  }
+  bytes = b;
+  return 0;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+int Field::alloc(int st, int64_t n) {
+  storage = st;
+  nelem = n;
+  int rc = buf.ensure((size_t)n * 12 * spin_elem_bytes(st));
+  if (rc) return rc;
+  if (st == STORE_H) return nrm.ensure((size_t)n * sizeof(float));
+  return 0;
+}
+
+FieldRef Field::ref() const {
+  FieldRef r;
+  r.data = buf.p;
+  r.norm = storage == STORE_H ? (float*)nrm.p : nullptr;
+  r.plane = nelem;
+  return r;
+}
+
+FieldRef null_field() {
+  FieldRef r;
+  r.data = nullptr;
+  r.norm = nullptr;
+  r.plane = 0;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+Lattice::Lattice() {
+  for (int k = 0; k < 4; k++) gauge_valid[k] = false;
+  // every slot exists up front: references handed out by field() stay valid
+  scratch.resize(S_NSLOTS);
+}
+
+Lattice::~Lattice() {
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Lattice::init(const int node_sites[4], int ls_, const int bc[4]) {
+  for (int mu = 0; mu < 4; mu++) {
+    if (node_sites[mu] <= 0 || (node_sites[mu] & 1)) return CPS_HIP_ERR_ARG;
+    g.L[mu] = node_sites[mu];
+    g.bc[mu] = bc ? (bc[mu] ? 1 : 0) : 0;
+  }
+  if (ls_ < 1) return CPS_HIP_ERR_ARG;
+  ls = ls_;
+  g.Vcb = g.L[0] * g.L[1] * g.L[2] * g.L[3] / 2;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return CPS_HIP_ERR_NODEV;
+  }
+  if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return CPS_HIP_ERR_RUNTIME;
+  return 0;
+}
+
+long long Lattice::device_bytes() const {
+  long long b = (long long)gauge_aos.bytes + partial.bytes + scal.bytes + io.bytes + io2.bytes;
+  for (int k = 0; k < 4; k++) b += gauge[k].bytes;
+  for (auto& f : scratch) b += f.buf.bytes + f.nrm.bytes;
+  b += clover.bytes;
+  return b;
+}
+
+int Lattice::import_gauge(const double* u_host) {
+  const size_t n = (size_t)72 * 2 * g.Vcb;
+  int rc = gauge_aos.ensure(n * sizeof(double));
+  if (rc) return rc;
+  CPSHIP_CHECK(hipMemcpyAsync(gauge_aos.p, u_host, n * sizeof(double), hipMemcpyHostToDevice, stream));
+  for (int k = 0; k < 4; k++) gauge_valid[k] = false;
+  rc = ensure_gauge(LINK_D18);
+  if (rc) return rc;
+  CPSHIP_CHECK(hipStreamSynchronize(stream));
+  has_gauge = true;
+  gauge_generation++;
+  return 0;
+}
+
+int Lattice::ensure_gauge(int link_storage) {
+  if (gauge_valid[link_storage]) return 0;
+  const size_t planes = link_planes(link_storage);
+  const int64_t plane = (int64_t)8 * g.Vcb;
+  int rc = gauge[link_storage].ensure(planes * plane * link_elem_bytes(link_storage));
+  if (rc) return rc;
+  GaugeRef gr;
+  gr.data = gauge[link_storage].p;
+  gr.plane = plane;
+  CPSHIP_CHECK(pack_gauge(link_storage, (const double*)gauge_aos.p, gr, g.Vcb, stream));
+  gauge_valid[link_storage] = true;
+  return 0;
+}
+
+GaugeRef Lattice::gauge_ref(int link_storage) const {
+  GaugeRef gr;
+  gr.data = gauge[link_storage].p;
+  gr.plane = (int64_t)8 * g.Vcb;
+  return gr;
+}
+
+Field& Lattice::field(int slot, int storage, int64_t nelem) {
+  Field& f = scratch[slot];   // slot < S_NSLOTS by construction
+  if (f.storage != storage || f.nelem != nelem || !f.buf.p) {
+    if (f.storage != storage) {
+      f.buf.release();
+      f.nrm.release();
+    }
+    last_rc = f.alloc(storage, nelem);
+  }
+  return f;
+}
+
+int Lattice::ensure_partial(int64_t slots) {
+  return partial.ensure((size_t)std::max<int64_t>(slots, 1) * sizeof(double));
+}
+
+// ---------------------------------------------------------------------------
+// Operator description and application
+
+int op_layout(const cps_hip_op* op, int ls, OpLayout* o, const Geom& g) {
+  if (!op) return CPS_HIP_ERR_ARG;
+  o->kind = op->kind;
+  switch (op->kind) {
+    case CPS_HIP_OP_WILSON:
+    case CPS_HIP_OP_CLOVER:
+      o->map = MAP_4D;
+      o->nb = 1;
+      o->nsites = g.Vcb;
+      o->per_rhs = g.Vcb;
+      break;
+    case CPS_HIP_OP_DWF:
+      if (ls < 2 || (ls & 1)) return CPS_HIP_ERR_ARG;
+      o->map = MAP_DWF5;
+      o->nb = ls / 2;
+      o->nsites = 2 * g.Vcb;
+      o->per_rhs = (int64_t)g.Vcb * ls;
+      break;
+    case CPS_HIP_OP_MOBIUS:
+      if (ls < 2) return CPS_HIP_ERR_ARG;
+      o->map = MAP_4DPREC;
+      o->nb = ls;
+      o->nsites = g.Vcb;
+      o->per_rhs = (int64_t)g.Vcb * ls;
+      break;
+    default:
+      return CPS_HIP_ERR_ARG;
+  }
+  return 0;
+}
+
+double op_kappa(const cps_hip_op* op) {
+  if (op->kind == CPS_HIP_OP_DWF) return 1.0 / (2 * (4 + op->a5_inv - op->m5));  // dwf_init.C:97-98
+  return 1.0 / (2.0 * (op->mass + 4.0));                                         // d_op_wilson.C:136
+}
+
+// out = xcoef*X + scale*D in, D the 4-D (Wilson) or 5-D (DWF) hop whose
+// output lives on `target` (Wilson: 4-D parity; DWF: 5-D checkerboard).
+int hop(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
+        double scale, const FieldRef& X, double xcoef, double* partial, int nrhs) {
+  OpLayout o;
+  int rc = op_layout(op, L.ls, &o, L.g);
+  if (rc) return rc;
+  HopParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.geom = L.g;
+  P.kind = (op->kind == CPS_HIP_OP_DWF) ? KIND_DWF : KIND_WILSON;
+  P.prec = hprec;
+  P.dag = dag;
+  P.nb = o.nb;
+  P.nrhs = nrhs;
+  P.nsites_out = o.nsites;
+  P.out_parity = target;
+  P.out_cb5 = target;
+  P.ls = L.ls;
+  P.block = pick_block((long long)o.nsites * o.nb);
+  P.link_bc_sign = (hprec != PREC_D_D18);
+  P.two_over_a5 = 2.0 * op->a5_inv;
+  P.neg_mass_two_over_a5 = -2.0 * op->mass * op->a5_inv;
+  P.scale = scale;
+  P.xcoef = xcoef;
+  P.in = in;
+  P.out = out;
+  P.X = X;
+  int lstore = hprec == PREC_D_D18 ? LINK_D18 : hprec == PREC_F_F12 ? LINK_F12 : hprec == PREC_H_H12 ? LINK_H12 : LINK_B12;
+  rc = L.ensure_gauge(lstore);
+  if (rc) return rc;
+  P.gauge = L.gauge_ref(lstore);
+  P.partial = partial;
+  CPSHIP_CHECK(launch_hop(P, L.stream));
+  return 0;
+}
+
+int hop_blocks_per_rhs(Lattice& L, const cps_hip_op* op) {
+  OpLayout o;
+  if (op_layout(op, L.ls, &o, L.g)) return 1;
+  const long long n = (long long)o.nsites * o.nb;
+  return (int)(n / pick_block(n));
+}
+
+// out = M in (dag 0) or M^dagger in (dag 1), M = 1 - kappa^2 D D
+// (wilson_m.C:73-91 / dwf_m.C:31-62); partial receives |out|^2 per block.
+int apply_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, const FieldRef& t,
+            int dag, double* partial, int nrhs) {
+  const double k = op_kappa(op);
+  int rc = hop(L, op, hprec, t, in, 0, dag, 1.0, null_field(), 0.0, nullptr, nrhs);
+  if (rc) return rc;
+  return hop(L, op, hprec, out, t, 1, dag, -k * k, in, 1.0, partial, nrhs);
+}
+
+// MatPcDagMatPc: out = M^dag M in; partial (optional) = |M in|^2 per block
+int apply_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs) {
+  if (op->kind == CPS_HIP_OP_MOBIUS) return mobius_mdagm(L, op, hprec, out, in, t1, t2, partial, nrhs);
+  if (op->kind == CPS_HIP_OP_CLOVER) return clover_mdagm(L, op, hprec, out, in, t1, t2, partial, nrhs);
+  int rc = apply_m(L, op, hprec, t2, in, t1, 0, partial, nrhs);
+  if (rc) return rc;
+  return apply_m(L, op, hprec, out, t2, t1, 1, nullptr, nrhs);
+}
+
+int apply_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                const FieldRef& t1, const FieldRef& t2, int dag, int nrhs) {
+  if (op->kind == CPS_HIP_OP_MOBIUS) return mobius_m(L, op, hprec, out, in, t1, t2, dag, nullptr, nrhs);
+  if (op->kind == CPS_HIP_OP_CLOVER) return clover_matpc(L, op, hprec, out, in, t1, dag, nullptr, nrhs);
+  return apply_m(L, op, hprec, out, in, t1, dag, nullptr, nrhs);
+}
+
+// ---------------------------------------------------------------------------
+// host <-> device vector transfer in CPS order
+
+int upload(Lattice& L, const double* host, Field& f, const OpLayout& o, int cb5, int nrhs, int slot_io) {
+  const int64_t n = o.per_rhs * nrhs;
+  DevBuf& io = slot_io ? L.io2 : L.io;
+  int rc = io.ensure((size_t)n * 24 * sizeof(double));
+  if (rc) return rc;
+  CPSHIP_CHECK(hipMemcpyAsync(io.p, host, (size_t)n * 24 * sizeof(double), hipMemcpyHostToDevice, L.stream));
+  CPSHIP_CHECK(pack_spinor(f.storage, (const double*)io.p, f.ref(), o.map, L.g.Vcb, L.ls, cb5, n, L.stream));
+  return 0;
+}
+
+int download(Lattice& L, const Field& f, double* host, const OpLayout& o, int cb5, int nrhs) {
+  const int64_t n = o.per_rhs * nrhs;
+  int rc = L.io.ensure((size_t)n * 24 * sizeof(double));
+  if (rc) return rc;
+  CPSHIP_CHECK(unpack_spinor(f.storage, f.ref(), (double*)L.io.p, o.map, L.g.Vcb, L.ls, cb5, n, L.stream));
+  CPSHIP_CHECK(hipMemcpyAsync(host, L.io.p, (size_t)n * 24 * sizeof(double), hipMemcpyDeviceToHost, L.stream));
+  CPSHIP_CHECK(hipStreamSynchronize(L.stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// CG scalar block
+
+int CGState::alloc(Lattice& L, int nrhs_, int hist_len_) {
+  nrhs = nrhs_;
+  const size_t nd = 10, ni = 4;
+  const size_t hist = (size_t)nrhs * hist_len_;
+  int rc = L.scal.ensure((nd * nrhs + hist) * sizeof(double) + ni * nrhs * sizeof(int) + 256);
+  if (rc) return rc;
+  double* d = (double*)L.scal.p;
+  s.rr = d + 0 * nrhs;
+  s.rr_prev = d + 1 * nrhs;
+  s.d = d + 2 * nrhs;
+  s.alpha = d + 3 * nrhs;
+  s.beta = d + 4 * nrhs;
+  s.stp = d + 5 * nrhs;
+  s.src_norm = d + 6 * nrhs;
+  s.rsd = d + 7 * nrhs;
+  rr_true = d + 8 * nrhs;
+  s.rr_upd = nullptr;          // set by the mixed solver
+  upd_store = d + 9 * nrhs;
+  s.delta2 = 0.0;
+  s.hist = hist_len_ ? d + nd * nrhs : nullptr;
+  s.hist_len = hist_len_ ? hist_len_ : 1;
+  int* ip = (int*)(d + nd * nrhs + hist);
+  s.active = ip;
+  s.iters = ip + nrhs;
+  s.status = ip + 2 * nrhs;
+  mask = ip + 3 * nrhs;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 CG: DiracOp::InvCgShift without shift, batched over nrhs.
+
+
+// CPS_HIP_VERBOSE=1 prints the residual history to stderr (the VRB.Flow
+// lines of inv_cg.C:352-353)
+static bool verbose() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("CPS_HIP_VERBOSE");
+    v = (e && *e && *e != '0') ? 1 : 0;
+  }
+  return v == 1;
+}
+
+static int sync_read(Lattice& L, void* dst, const void* src, size_t bytes) {
+  CPSHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, L.stream));
+  CPSHIP_CHECK(hipStreamSynchronize(L.stream));
+  return 0;
+}
+
+static int auto_check_every(const OpLayout& o, int nrhs) {
+  const double elems = (double)o.per_rhs * nrhs;
+  if (elems < 1e5) return 32;
+  if (elems < 1e6) return 16;
+  return 8;
+}
+
+int cg_double_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int nrhs, const cps_hip_cg_arg* arg,
+                     std::vector<int>& iters, std::vector<int>& status, std::vector<double>& true_res,
+                     std::vector<double>& src_norm) {
+  OpLayout o;
+  int rc = op_layout(op, L.ls, &o, L.g);
+  if (rc) return rc;
+  const int64_t n = o.per_rhs * nrhs;
+  Field& r = L.field(S_R, STORE_D, n);
+  Field& p = L.field(S_P, STORE_D, n);
+  Field& ap = L.field(S_AP, STORE_D, n);
+  Field& t1 = L.field(S_T1, STORE_D, n);
+  Field& t2 = L.field(S_T2, STORE_D, n);
+  if (L.last_rc) return L.last_rc;
+  const int bsb = pick_block(o.per_rhs);
+  const int bpr_blas = (int)(o.per_rhs / bsb);
+  const int bpr_hop = hop_blocks_per_rhs(L, op);
+  rc = L.ensure_partial((int64_t)std::max(bpr_blas, bpr_hop) * nrhs);
+  if (rc) return rc;
+  double* part = (double*)L.partial.p;
+  CGState cg;
+  rc = cg.alloc(L, nrhs, 0);
+  if (rc) return rc;
+  cg.s.max_itr = arg->max_num_iter - 1;
+  {
+    std::vector<double> rsd(nrhs, arg->stop_rsd);
+    CPSHIP_CHECK(hipMemcpyAsync(cg.s.rsd, rsd.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, L.stream));
+  }
+  // |src|^2 and stp_cnd (inv_cg.C:166-174)
+  CPSHIP_CHECK(blas_norm(STORE_D, src.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
+  CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_SRC_NORM, cg.s, L.stream));
+  // r = src - MdagM x ; p = r ; |r|^2 (inv_cg.C:248-269)
+  rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), x.ref(), t1.ref(), t2.ref(), nullptr, nrhs);
+  if (rc) return rc;
+  CPSHIP_CHECK(blas_sub_norm_d(src.ref(), ap.ref(), r.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
+  CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_CG_INIT, cg.s, L.stream));
+  CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), STORE_D, p.ref(), n, L.stream));
+
+  const int chunk = arg->check_every > 0 ? arg->check_every : auto_check_every(o, nrhs);
+  std::vector<int> act(nrhs);
+  int done_iters = 0;
+  const int max_itr = std::max(arg->max_num_iter - 1, 0);
+  rc = sync_read(L, act.data(), cg.s.active, nrhs * sizeof(int));
+  if (rc) return rc;
+  auto any_active = [&]() {
+    for (int k = 0; k < nrhs; k++)
+      if (act[k]) return true;
+    return false;
+  };
+  while (any_active() && done_iters < max_itr) {
+    const int todo = std::min(chunk, max_itr - done_iters);
+    for (int it = 0; it < todo; it++) {
+      // mmp = MdagM p, d = |M p|^2 (inv_cg.C:302)
+      rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), p.ref(), t1.ref(), t2.ref(), part, nrhs);
+      if (rc) return rc;
+      CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nrhs, nullptr, RED_CG_ALPHA, cg.s, L.stream));
+      CPSHIP_CHECK(blas_cg_xr_d(x.ref(), p.ref(), r.ref(), ap.ref(), cg.s, o.per_rhs, nrhs, bsb, part, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_CG_BETA, cg.s, L.stream));
+      CPSHIP_CHECK(blas_xpay(STORE_D, p.ref(), r.ref(), cg.s.beta, cg.s.active, o.per_rhs, nrhs, bsb, L.stream));
+    }
+    done_iters += todo;
+    rc = sync_read(L, act.data(), cg.s.active, nrhs * sizeof(int));
+    if (rc) return rc;
+  }
+  // true residual (inv_cg.C:403-417)
+  rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), x.ref(), t1.ref(), t2.ref(), nullptr, nrhs);
+  if (rc) return rc;
+  CPSHIP_CHECK(blas_sub_norm_d(src.ref(), ap.ref(), r.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
+  double* trn = cg.s.d;  // reuse as output slot
+  CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, trn, RED_PLAIN, cg.s, L.stream));
+  std::vector<double> tr(nrhs), sn(nrhs);
+  iters.resize(nrhs);
+  status.resize(nrhs);
+  rc = sync_read(L, tr.data(), trn, nrhs * sizeof(double));
+  if (rc) return rc;
+  rc = sync_read(L, sn.data(), cg.s.src_norm, nrhs * sizeof(double));
+  if (rc) return rc;
+  rc = sync_read(L, iters.data(), cg.s.iters, nrhs * sizeof(int));
+  if (rc) return rc;
+  rc = sync_read(L, status.data(), cg.s.status, nrhs * sizeof(int));
+  if (rc) return rc;
+  true_res.resize(nrhs);
+  src_norm = sn;
+  for (int k = 0; k < nrhs; k++) {
+    true_res[k] = std::sqrt(tr[k] / sn[k]);
+    if (status[k] == CG_STATUS_RUNNING) status[k] = CG_STATUS_MAXITER;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision CG with FP64 reliable updates.
+//
+// Sloppy iteration: FP32 arithmetic, p / Ap / operator temporaries stored in
+// the sloppy spinor format, the sloppy residual rs in FP32, the partial
+// solution xs in FP32.  Every `reliable_every` iterations, and whenever the
+// sloppy residual says a rhs has converged, the FP64-critical steps run:
+//   x += xs (FP64 accumulate), r = src - MdagM x (FP64 operator and links),
+//   rs = r, xs = 0;
+// a rhs stops only when that FP64 residual meets |r|^2 <= stop_rsd^2 |src|^2.
+
+static void mixed_formats(int prec, int* sp_store, int* hprec) {
+  switch (prec) {
+    case CPS_HIP_PREC_MIXED_SINGLE: *sp_store = STORE_F; *hprec = PREC_F_F12; break;
+    case CPS_HIP_PREC_MIXED_BF16: *sp_store = STORE_H; *hprec = PREC_H_B12; break;
+    default: *sp_store = STORE_H; *hprec = PREC_H_H12; break;
+  }
+}
+
+// After a reliable update (rr_true = FP64 |src - MdagM x|^2 per rhs):
+//  - a rhs whose FP64 residual meets the stopping rule is finished;
+//  - a rhs that paused inside the last chunk (sloppy convergence or the
+//    delta trigger) skipped its direction update there, so it gets it now,
+//    p = r + beta p with beta = |r|^2 / |r_prev|^2 from the FP64 residual
+//    (mask = 1 marks those rhs for the xpay launched next);
+//  - every unfinished rhs continues from the FP64 residual norm.
+__global__ void k_mixed_after_update(CGScalars cg, const double* rr_true, int* mask, int nrhs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nrhs) return;
+  mask[k] = 0;
+  const int st = cg.status[k];
+  if (st == CG_STATUS_D_ZERO || st == CG_STATUS_MAXITER) return;
+  if (st == CG_STATUS_CONVERGED && !cg.active[k] && cg.iters[k] == 0 && rr_true[k] <= cg.stp[k]) return;
+  const double v = rr_true[k];
+  cg.rr_upd[k] = v;
+  if (v <= cg.stp[k]) {
+    cg.rr[k] = v;
+    cg.active[k] = 0;
+    cg.status[k] = CG_STATUS_CONVERGED;
+    return;
+  }
+  if (!cg.active[k]) {
+    // paused during the chunk: finish its interrupted iteration
+    cg.beta[k] = v / cg.rr_prev[k];
+    mask[k] = 1;
+  }
+  cg.rr[k] = v;
+  cg.active[k] = 1;
+  cg.status[k] = CG_STATUS_RUNNING;
+}
+
+int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int nrhs, const cps_hip_cg_arg* arg,
+                    std::vector<int>& iters, std::vector<int>& status, std::vector<double>& true_res,
+                    std::vector<double>& src_norm, std::vector<int>& n_updates) {
+  OpLayout o;
+  int rc = op_layout(op, L.ls, &o, L.g);
+  if (rc) return rc;
+  int sps, hprec;
+  mixed_formats(arg->precision, &sps, &hprec);
+  const int64_t n = o.per_rhs * nrhs;
+  Field& r = L.field(S_R, STORE_D, n);
+  Field& ap = L.field(S_AP, STORE_D, n);
+  Field& t1 = L.field(S_T1, STORE_D, n);
+  Field& t2 = L.field(S_T2, STORE_D, n);
+  Field& xs = L.field(S_XS, STORE_F, n);
+  Field& rs = L.field(S_RS, STORE_F, n);
+  Field& ps = L.field(S_PS, sps, n);
+  Field& aps = L.field(S_APS, sps, n);
+  Field& t1s = L.field(S_T1S, sps, n);
+  Field& t2s = L.field(S_T2S, sps, n);
+  if (L.last_rc) return L.last_rc;
+  const int bsb = pick_block(o.per_rhs);
+  const int bpr_blas = (int)(o.per_rhs / bsb);
+  const int bpr_hop = hop_blocks_per_rhs(L, op);
+  rc = L.ensure_partial((int64_t)std::max(bpr_blas, bpr_hop) * nrhs);
+  if (rc) return rc;
+  double* part = (double*)L.partial.p;
+  CGState cg;
+  rc = cg.alloc(L, nrhs, 0);
+  if (rc) return rc;
+  const int max_itr = std::max(arg->max_num_iter - 1, 0);
+  cg.s.max_itr = max_itr;
+  {
+    std::vector<double> rsd(nrhs, arg->stop_rsd);
+    CPSHIP_CHECK(hipMemcpyAsync(cg.s.rsd, rsd.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, L.stream));
+  }
+  double* rr_true = cg.rr_true;
+  cg.s.rr_upd = cg.upd_store;
+  {
+    const double delta = arg->reliable_delta > 0 ? arg->reliable_delta : 0.1;
+    cg.s.delta2 = delta * delta;
+  }
+
+  CPSHIP_CHECK(blas_norm(STORE_D, src.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
+  CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_SRC_NORM, cg.s, L.stream));
+  // FP64 initial residual
+  rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), x.ref(), t1.ref(), t2.ref(), nullptr, nrhs);
+  if (rc) return rc;
+  CPSHIP_CHECK(blas_sub_norm_d(src.ref(), ap.ref(), r.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
+  CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, cg.s.rr_upd, RED_CG_INIT, cg.s, L.stream));
+  CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), STORE_F, rs.ref(), n, L.stream));
+  CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), sps, ps.ref(), n, L.stream));
+  CPSHIP_CHECK(blas_zero(STORE_F, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
+
+  const int chunk = arg->check_every > 0 ? arg->check_every : auto_check_every(o, nrhs);
+  const int every = arg->reliable_every > 0 ? arg->reliable_every : 100;
+  std::vector<int> act(nrhs), st(nrhs);
+  n_updates.assign(nrhs, 0);
+  rc = sync_read(L, act.data(), cg.s.active, nrhs * sizeof(int));
+  if (rc) return rc;
+  std::vector<int> live = act;  // rhs not yet converged on the FP64 residual
+  int done = 0, since = 0;
+  auto any = [&](const std::vector<int>& v) {
+    for (int k = 0; k < nrhs; k++)
+      if (v[k]) return true;
+    return false;
+  };
+  while (any(live) && done < max_itr) {
+    const int todo = std::min(std::min(chunk, max_itr - done), every - since);
+    for (int it = 0; it < todo; it++) {
+      rc = apply_mdagm(L, op, hprec, aps.ref(), ps.ref(), t1s.ref(), t2s.ref(), part, nrhs);
+      if (rc) return rc;
+      CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nrhs, nullptr, RED_CG_ALPHA, cg.s, L.stream));
+      CPSHIP_CHECK(blas_cg_xr_mixed(STORE_F, xs.ref(), sps, ps.ref(), STORE_F, rs.ref(), sps, aps.ref(), cg.s,
+                                    o.per_rhs, nrhs, bsb, part, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_CG_BETA, cg.s, L.stream));
+      CPSHIP_CHECK(blas_xpay_mixed(sps, ps.ref(), STORE_F, rs.ref(), cg.s.beta, cg.s.active, o.per_rhs, nrhs, bsb,
+                                   L.stream));
+    }
+    done += todo;
+    since += todo;
+    rc = sync_read(L, act.data(), cg.s.active, nrhs * sizeof(int));
+    if (rc) return rc;
+    rc = sync_read(L, st.data(), cg.s.status, nrhs * sizeof(int));
+    if (rc) return rc;
+    bool paused = false;   // a rhs stopped on the sloppy residual or the delta trigger
+    for (int k = 0; k < nrhs; k++)
+      if (live[k] && !act[k]) paused = true;
+    if (verbose()) {
+      std::vector<double> rr(nrhs), al(nrhs), be(nrhs), dd(nrhs);
+      sync_read(L, rr.data(), cg.s.rr, nrhs * sizeof(double));
+      sync_read(L, al.data(), cg.s.alpha, nrhs * sizeof(double));
+      sync_read(L, be.data(), cg.s.beta, nrhs * sizeof(double));
+      sync_read(L, dd.data(), cg.s.d, nrhs * sizeof(double));
+      fprintf(stderr, "cps_hip mixed CG iter %d: |r_s|^2 %.6e alpha %.6e beta %.6e d %.6e active %d\n", done, rr[0],
+              al[0], be[0], dd[0], act[0]);
+    }
+    if (since >= every || paused || done >= max_itr) {
+      // ---- FP64-critical steps: accumulate the solution, recompute r ----
+      CPSHIP_CHECK(blas_axpy(STORE_F, xs.ref(), STORE_D, x.ref(), nullptr, 1.0, o.per_rhs, nrhs, bsb, L.stream));
+      CPSHIP_CHECK(blas_zero(STORE_F, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
+      rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), x.ref(), t1.ref(), t2.ref(), nullptr, nrhs);
+      if (rc) return rc;
+      CPSHIP_CHECK(blas_sub_norm_d(src.ref(), ap.ref(), r.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, rr_true, RED_PLAIN, cg.s, L.stream));
+      CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), STORE_F, rs.ref(), n, L.stream));
+      hipLaunchKernelGGL(k_mixed_after_update, dim3((nrhs + 63) / 64), dim3(64), 0, L.stream, cg.s, rr_true, cg.mask,
+                         nrhs);
+      CPSHIP_CHECK(hipGetLastError());
+      // the direction update that the paused rhs skipped
+      CPSHIP_CHECK(blas_xpay_mixed(sps, ps.ref(), STORE_F, rs.ref(), cg.s.beta, cg.mask, o.per_rhs, nrhs, bsb,
+                                   L.stream));
+      for (int k = 0; k < nrhs; k++)
+        if (live[k]) n_updates[k]++;
+      since = 0;
+      rc = sync_read(L, act.data(), cg.s.active, nrhs * sizeof(int));
+      if (rc) return rc;
+      rc = sync_read(L, st.data(), cg.s.status, nrhs * sizeof(int));
+      if (rc) return rc;
+      for (int k = 0; k < nrhs; k++)
+        if (live[k] && st[k] == CG_STATUS_CONVERGED) live[k] = 0;
+      if (verbose()) {
+        std::vector<double> rt(nrhs);
+        sync_read(L, rt.data(), rr_true, nrhs * sizeof(double));
+        fprintf(stderr, "cps_hip mixed CG reliable update at %d: true |r|^2 %.6e\n", done, rt[0]);
+      }
+    }
+  }
+  // final FP64 true residual of MdagM (inv_cg.C:403-417)
+  rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), x.ref(), t1.ref(), t2.ref(), nullptr, nrhs);
+  if (rc) return rc;
+  CPSHIP_CHECK(blas_sub_norm_d(src.ref(), ap.ref(), r.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
+  CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, rr_true, RED_PLAIN, cg.s, L.stream));
+  std::vector<double> tr(nrhs), sn(nrhs);
+  iters.resize(nrhs);
+  status.resize(nrhs);
+  rc = sync_read(L, tr.data(), rr_true, nrhs * sizeof(double));
+  if (rc) return rc;
+  rc = sync_read(L, sn.data(), cg.s.src_norm, nrhs * sizeof(double));
+  if (rc) return rc;
+  rc = sync_read(L, iters.data(), cg.s.iters, nrhs * sizeof(int));
+  if (rc) return rc;
+  rc = sync_read(L, status.data(), cg.s.status, nrhs * sizeof(int));
+  if (rc) return rc;
+  true_res.resize(nrhs);
+  src_norm = sn;
+  for (int k = 0; k < nrhs; k++) {
+    true_res[k] = std::sqrt(tr[k] / sn[k]);
+    if (status[k] == CG_STATUS_RUNNING) status[k] = CG_STATUS_MAXITER;
+  }
+  return 0;
+}
+
+}  // namespace cpship

@@ -1,0 +1,181 @@
+/*
+ * cps_hip.h -- C ABI of the MI355X (gfx950) implementation of the CPS
+ * even-odd preconditioned Wilson / domain-wall / Moebius / clover operators
+ * and their conjugate-gradient inverter.
+ *
+ * The entry points mirror the CPS fermion "library" functions that the
+ * DiracOp classes call (include/util/wilson.h:187-284, include/util/dwf.h:
+ * 97-248 of the CPS source tree cps_pp/) and DiracOp::InvCg
+ * (src/util/dirac_op/d_op_base/comsrc/inv_cg.C:84-454).  Vectors passed as
+ * `double *` are host arrays in the CPS WILSON order the reference functions
+ * take:
+ *   4-D checkerboard spinor : [site_cb][spin 4][colour 3][re,im]
+ *   full 4-D spinor         : [odd checkerboard][even checkerboard]
+ *   5-D DWF checkerboard    : Ls slices of V/2 sites, slice s of checkerboard
+ *                             cb at 4-D parity (s+cb)%2
+ *   5-D Moebius (4-D prec.) : Ls slices of V/2 sites of one 4-D parity
+ *   gauge                   : [parity][site_cb][mu][col][row][re,im]
+ *                             (boundary signs already applied, as the
+ *                             DiracOp constructor leaves it)
+ * Multi-RHS variants take nrhs such vectors back to back.
+ *
+ * Every function returns 0 on success or a negative CPS_HIP_ERR_* code; no
+ * function aborts the process (the C++ DiracOp front end maps codes to the
+ * CPS ERR.* behaviour).  A handle is not thread safe; distinct handles may be
+ * used from distinct threads.
+ */
+#ifndef CPS_HIP_H
+#define CPS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPS_HIP_OK 0
+#define CPS_HIP_ERR_ARG (-1)       /* bad argument (cb/dag not 0/1, sizes) */
+#define CPS_HIP_ERR_RUNTIME (-2)   /* HIP runtime failure */
+#define CPS_HIP_ERR_NOGAUGE (-3)   /* no gauge field resident */
+#define CPS_HIP_ERR_D_ZERO (-4)    /* CG: <p,Ap> = 0 (inv_cg.C:326-329) */
+#define CPS_HIP_ERR_NOMEM (-5)     /* device allocation failed / over budget */
+#define CPS_HIP_ERR_NODEV (-6)     /* no HIP device */
+
+/* operator kinds */
+#define CPS_HIP_OP_WILSON 0
+#define CPS_HIP_OP_DWF 1
+#define CPS_HIP_OP_MOBIUS 2
+#define CPS_HIP_OP_CLOVER 3
+
+/* solver precision modes */
+#define CPS_HIP_PREC_DOUBLE 0        /* plain FP64 CG (the CPS algorithm) */
+#define CPS_HIP_PREC_MIXED_HALF 1    /* FP32 math, FP16 spinors + FP16 12-real links */
+#define CPS_HIP_PREC_MIXED_BF16 2    /* FP32 math, FP16 spinors + BF16 12-real links */
+#define CPS_HIP_PREC_MIXED_SINGLE 3  /* FP32 math, FP32 spinors + FP32 12-real links */
+
+typedef struct cps_hip_lattice cps_hip_lattice;
+
+/* Operator parameters: the union of what DiracOpWilson (d_op_wilson.C:134),
+ * DiracOpDwf (dwf_init.C:97), DiracOpMobius (d_op_mobius.C:106-122) and
+ * DiracOpClover (d_op_clover.C:107-122) read from CgArg and GJP. */
+typedef struct {
+  int kind;          /* CPS_HIP_OP_* */
+  double mass;       /* CgArg.mass */
+  double m5;         /* GJP.DwfHeight() */
+  double a5_inv;     /* GJP.DwfA5Inv() */
+  double mobius_b;   /* GJP.Mobius_b() */
+  double mobius_c;   /* GJP.Mobius_c() */
+  double csw;        /* GJP.CloverCoeff() */
+} cps_hip_op;
+
+/* CgArg subset used by InvCg (include/input/cg_arg.x:2-26) plus the
+ * mixed-precision controls. */
+typedef struct {
+  int max_num_iter;      /* CgArg.max_num_iter */
+  double stop_rsd;       /* CgArg.stop_rsd */
+  int precision;         /* CPS_HIP_PREC_* */
+  int reliable_every;    /* mixed: FP64 reliable update at least every N iterations */
+  double reliable_delta; /* mixed: also when |r_sloppy| < delta * |r| at last update */
+  int check_every;       /* host convergence polling interval (iterations), 0 = auto */
+} cps_hip_cg_arg;
+
+/* Per-rhs result of a solve. */
+typedef struct {
+  int iterations;        /* InvCg return value: iterations + 1 (inv_cg.C:446) */
+  int status;            /* 1 converged, 2 max iterations, 3 d == 0 */
+  double true_res;       /* |src - MdagM sol| / |src| in FP64 (inv_cg.C:403-417) */
+  double src_norm_sq;
+  int reliable_updates;  /* mixed precision only */
+  double seconds;        /* wall time of the batched solve (same for every rhs) */
+} cps_hip_cg_result;
+
+/* ---- lattice handle ------------------------------------------------------
+ * Replaces wilson_init/dwf_init (wilson_init.C:52-112, dwf_init.C:41-99): it
+ * holds the local geometry, the device scratch fields and the resident gauge.
+ * bc[mu] = 1 marks an antiperiodic direction (GJP.Bc(mu) == BND_CND_APRD);
+ * the sign itself travels in the gauge field as in CPS, and bc is used only
+ * to rebuild the third row of compressed (12-real) links.  ls may be 1 for
+ * 4-D operators. */
+int cps_hip_lattice_create(cps_hip_lattice **lat, const int node_sites[4], int ls, const int bc[4]);
+int cps_hip_lattice_destroy(cps_hip_lattice *lat);
+/* Upload the WILSON-order FP64 gauge field (72*V/2*2 reals) and build the
+ * device copies (FP64 18-real, and the 12-real low-precision copies on first
+ * use). */
+int cps_hip_gauge_import(cps_hip_lattice *lat, const double *u_wilson);
+/* Device bytes currently held by the handle. */
+long long cps_hip_lattice_device_bytes(const cps_hip_lattice *lat);
+
+/* ---- Wilson library (wilson.h:187-284) ---------------------------------
+ * u: the WILSON-order gauge; if non-NULL it is imported first (equivalent to
+ * cps_hip_gauge_import), if NULL the resident gauge is used. */
+/* wilson_dslash (wilson_dslash.C:173-515): chi on parity 1-cb from psi on cb */
+int cps_hip_wilson_dslash(cps_hip_lattice *lat, double *chi, const double *u, const double *psi, int cb, int dag);
+/* wilson_m / wilson_mdag (wilson_m.C:49-93, wilson_mdag.C:50-95) */
+int cps_hip_wilson_m(cps_hip_lattice *lat, double *chi, const double *u, const double *psi, double kappa);
+int cps_hip_wilson_mdag(cps_hip_lattice *lat, double *chi, const double *u, const double *psi, double kappa);
+/* wilson_mdagm (wilson_mdagm.C:83-164); mp_sq (may be NULL) = |M psi|^2 */
+int cps_hip_wilson_mdagm(cps_hip_lattice *lat, double *chi, const double *u, const double *psi, double *mp_sq,
+                         double kappa);
+
+/* ---- domain-wall library (dwf.h:97-248) --------------------------------- */
+/* dwf_dslash (dwf_dslash.C:32-50): 4-D hop on every slice + 5th-dim hop */
+int cps_hip_dwf_dslash(cps_hip_lattice *lat, double *out, const double *u, const double *in, double mass, int cb,
+                       int dag, double m5, double a5_inv);
+/* dwf_m / dwf_mdag / dwf_mdagm (dwf_m.C:31-62, dwf_mdag.C:32-62, dwf_mdagm.C:32-61) */
+int cps_hip_dwf_m(cps_hip_lattice *lat, double *out, const double *u, const double *in, double mass, double m5,
+                  double a5_inv);
+int cps_hip_dwf_mdag(cps_hip_lattice *lat, double *out, const double *u, const double *in, double mass, double m5,
+                     double a5_inv);
+int cps_hip_dwf_mdagm(cps_hip_lattice *lat, double *out, const double *u, const double *in, double *dot_prd,
+                      double mass, double m5, double a5_inv);
+
+/* ---- generic operator entry points (DiracOp*::MatPcDagMatPc / MatPc) -----
+ * For every kind: MatPcDagMatPc (dirac_op.h:262), with dot_prd = |MatPc in|^2
+ * per rhs (may be NULL).  Clover requires cps_hip_clover_build first. */
+int cps_hip_matpc_dag_matpc(cps_hip_lattice *lat, const cps_hip_op *op, double *out, const double *in,
+                            double *dot_prd, int nrhs);
+int cps_hip_matpc(cps_hip_lattice *lat, const cps_hip_op *op, double *out, const double *in, int dag, int nrhs);
+/* MatPcDagMatPc evaluated in the storage and arithmetic of a solver
+ * precision mode (CPS_HIP_PREC_*): the FP64 host input is converted to that
+ * format on the device, the operator runs there, and the result comes back
+ * as FP64.  This is the sloppy operator the mixed-precision CG iterates with;
+ * it has no counterpart in CPS (whose operators are FP64 only). */
+int cps_hip_matpc_dag_matpc_prec(cps_hip_lattice *lat, const cps_hip_op *op, int precision, double *out,
+                                 const double *in, double *dot_prd, int nrhs);
+
+/* Clover: build A_oo and A_ee^{-1} on the device from the resident gauge for
+ * the given kappa list (CloverMatChkb, d_op_clover_supp.C:492-540).  Solves
+ * with nrhs = n_kappa * k use clover field rhs / k. */
+int cps_hip_clover_build(cps_hip_lattice *lat, const double *kappas, int n_kappa, double csw);
+/* Export the packed clover matrices of kappa index k (72 reals per site,
+ * CPS clover_mat_mlt layout) of one checkerboard: cb 1 = A_oo, cb 0 = A_ee^{-1}. */
+int cps_hip_clover_export(cps_hip_lattice *lat, int k, int cb, double *A);
+
+/* ---- DiracOp::InvCg (inv_cg.C:84-454) -------------------------------------
+ * Solves MdagM sol = src for nrhs right-hand sides at once.  sol holds the
+ * initial guesses on entry.  With precision CPS_HIP_PREC_DOUBLE each rhs
+ * follows the reference algorithm (same recurrences, stopping rule
+ * |r|^2 <= stop_rsd^2 |src|^2 and FP64 true residual).  Mixed modes run the
+ * sloppy CG with FP64 reliable updates and stop on the FP64 residual. */
+int cps_hip_invcg(cps_hip_lattice *lat, const cps_hip_op *op, double *sol, const double *src, int nrhs,
+                  const cps_hip_cg_arg *arg, cps_hip_cg_result *res);
+
+/* ---- MatInv (d_op_wilson.C:257-334, d_op_dwf.C:316-509) ------------------
+ * Full-vector solve M out = in by even-odd preconditioning + InvCg; in and
+ * out are full WILSON-order vectors ([odd][even]).  in is preserved. */
+int cps_hip_matinv(cps_hip_lattice *lat, const cps_hip_op *op, double *out, const double *in, int nrhs,
+                   const cps_hip_cg_arg *arg, cps_hip_cg_result *res);
+
+/* ---- measurement helpers ------------------------------------------------ */
+/* Time `reps` applications of the single-parity hop kernel of an operator
+ * kind on resident random data, in the given precision; returns the mean
+ * kernel time in seconds (HIP events on the launch stream) and the
+ * algorithmic bytes moved per launch. */
+int cps_hip_bench_hop(cps_hip_lattice *lat, int kind, int precision, int nrhs, int reps, double *seconds,
+                      double *bytes);
+
+int cps_hip_device_count(void);
+const char *cps_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

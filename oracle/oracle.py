@@ -1,0 +1,356 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module as the checker.  The product
+(cps_public_amd) never imports it.
+
+All arrays are float64 numpy arrays in the CPS layouts documented in
+cps_oracle.h; functions return new arrays and leave their inputs alone unless
+stated.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build(force=False):
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class OraDwf(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int * 4), ("ls", ctypes.c_int),
+                ("m5", ctypes.c_double), ("a5_inv", ctypes.c_double)]
+
+
+class OraMobius(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int * 4), ("ls", ctypes.c_int),
+                ("m5", ctypes.c_double), ("a5_inv", ctypes.c_double),
+                ("b", ctypes.c_double), ("c", ctypes.c_double)]
+
+
+def _declare(l):
+    vp = ctypes.c_void_p
+    sig = {
+        "ora_gauge_canon_to_wilson": [vp, vp, vp],
+        "ora_gauge_wilson_to_canon": [vp, vp, vp],
+        "ora_ferm_canon_to_wilson": [vp, vp, vp],
+        "ora_ferm_wilson_to_canon": [vp, vp, vp],
+        "ora_ferm5_canon_to_wilson": [vp, vp, vp, ctypes.c_int],
+        "ora_ferm5_wilson_to_canon": [vp, vp, vp, ctypes.c_int],
+        "ora_ferm5_canon_to_4dprec": [vp, vp, vp, ctypes.c_int],
+        "ora_ferm5_4dprec_to_canon": [vp, vp, vp, ctypes.c_int],
+        "ora_bond_cond": [vp, vp, vp],
+        "ora_wilson_dslash": [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp],
+        "ora_wilson_m": [vp, vp, vp, ctypes.c_double, vp, vp],
+        "ora_wilson_mdag": [vp, vp, vp, ctypes.c_double, vp, vp],
+        "ora_wilson_mdagm": [vp, vp, vp, vp, ctypes.c_double, vp, vp, vp],
+        "ora_dwf_dslash_4": [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp],
+        "ora_dwf_dslash_5_plus": [vp, vp, ctypes.c_double, ctypes.c_int, vp],
+        "ora_dwf_dslash": [vp, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, vp],
+        "ora_dwf_m": [vp, vp, vp, ctypes.c_double, vp, vp],
+        "ora_dwf_mdag": [vp, vp, vp, ctypes.c_double, vp, vp],
+        "ora_dwf_mdagm": [vp, vp, vp, vp, ctypes.c_double, vp, vp, vp],
+        "ora_mobius_dslash_4": [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_double],
+        "ora_mobius_m5inv": [vp, ctypes.c_double, ctypes.c_int, vp],
+        "ora_mobius_dslash_5_plus": [vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, vp],
+        "ora_mobius_m": [vp, vp, vp, ctypes.c_double, vp, vp],
+        "ora_mobius_mdag": [vp, vp, vp, ctypes.c_double, vp, vp, vp],
+        "ora_mobius_mdagm": [vp, vp, vp, vp, ctypes.c_double, vp, vp, vp, vp],
+        "ora_clover_mat_chkb": [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp],
+        "ora_clover_mat_mlt": [vp, vp, vp, ctypes.c_int],
+        "ora_clover_matpc": [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_int, vp, vp, vp],
+        "ora_clover_mdagm": [vp, vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, vp, vp],
+        "ora_wilson_invcg": [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp],
+        "ora_wilson_matinv": [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp],
+        "ora_dwf_invcg": [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp],
+        "ora_dwf_matinv": [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp],
+        "ora_mobius_invcg": [vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp],
+        "ora_clover_invcg": [vp, vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, vp, vp],
+    }
+    for name, args in sig.items():
+        f = getattr(l, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int if name.endswith(("invcg", "matinv")) else None
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data
+
+
+def _L(L):
+    arr = (ctypes.c_int * 4)(*[int(v) for v in L])
+    return arr
+
+
+def vol(L):
+    return int(L[0] * L[1] * L[2] * L[3])
+
+
+# ---- layout -------------------------------------------------------------
+def gauge_canon_to_wilson(u, L):
+    out = np.empty_like(u)
+    lib().ora_gauge_canon_to_wilson(_p(out), _p(u), _L(L))
+    return out
+
+
+def gauge_wilson_to_canon(u, L):
+    out = np.empty_like(u)
+    lib().ora_gauge_wilson_to_canon(_p(out), _p(u), _L(L))
+    return out
+
+
+def ferm_canon_to_wilson(f, L):
+    out = np.empty_like(f)
+    lib().ora_ferm_canon_to_wilson(_p(out), _p(f), _L(L))
+    return out
+
+
+def ferm_wilson_to_canon(f, L):
+    out = np.empty_like(f)
+    lib().ora_ferm_wilson_to_canon(_p(out), _p(f), _L(L))
+    return out
+
+
+def ferm5_canon_to_wilson(f, L, ls):
+    out = np.empty_like(f)
+    lib().ora_ferm5_canon_to_wilson(_p(out), _p(f), _L(L), ls)
+    return out
+
+
+def ferm5_wilson_to_canon(f, L, ls):
+    out = np.empty_like(f)
+    lib().ora_ferm5_wilson_to_canon(_p(out), _p(f), _L(L), ls)
+    return out
+
+
+def ferm5_canon_to_4dprec(f, L, ls):
+    out = np.empty_like(f)
+    lib().ora_ferm5_canon_to_4dprec(_p(out), _p(f), _L(L), ls)
+    return out
+
+
+def ferm5_4dprec_to_canon(f, L, ls):
+    out = np.empty_like(f)
+    lib().ora_ferm5_4dprec_to_canon(_p(out), _p(f), _L(L), ls)
+    return out
+
+
+def bond_cond(u_canon, L, bc):
+    u = u_canon.copy()
+    lib().ora_bond_cond(_p(u), _L(L), _L(bc))
+    return u
+
+
+# ---- Wilson -------------------------------------------------------------
+def half(L):
+    return 24 * vol(L) // 2
+
+
+def wilson_dslash(u, psi, cb, dag, L):
+    chi = np.zeros(half(L))
+    lib().ora_wilson_dslash(_p(chi), _p(u), _p(psi), cb, dag, _L(L))
+    return chi
+
+
+def wilson_m(u, psi, kappa, L):
+    chi = np.zeros(half(L))
+    t = np.zeros(half(L))
+    lib().ora_wilson_m(_p(chi), _p(u), _p(psi), kappa, _L(L), _p(t))
+    return chi
+
+
+def wilson_mdag(u, psi, kappa, L):
+    chi = np.zeros(half(L))
+    t = np.zeros(half(L))
+    lib().ora_wilson_mdag(_p(chi), _p(u), _p(psi), kappa, _L(L), _p(t))
+    return chi
+
+
+def wilson_mdagm(u, psi, kappa, L):
+    chi = np.zeros(half(L))
+    t1, t2 = np.zeros(half(L)), np.zeros(half(L))
+    mp = np.zeros(1)
+    lib().ora_wilson_mdagm(_p(chi), _p(u), _p(psi), _p(mp), kappa, _L(L), _p(t1), _p(t2))
+    return chi, float(mp[0])
+
+
+def wilson_invcg(u, src, mass, stop_rsd, max_iter, L, guess=None):
+    sol = np.zeros(half(L)) if guess is None else guess.copy()
+    tr = np.zeros(1)
+    it = lib().ora_wilson_invcg(_p(sol), _p(src), _p(u), mass, stop_rsd, max_iter, _L(L), _p(tr))
+    return sol, it, float(tr[0])
+
+
+def wilson_matinv(u, b, mass, stop_rsd, max_iter, L, guess=None):
+    out = np.zeros(2 * half(L)) if guess is None else guess.copy()
+    b = b.copy()
+    tr = np.zeros(1)
+    it = lib().ora_wilson_matinv(_p(out), _p(b), _p(u), mass, stop_rsd, max_iter, _L(L), _p(tr))
+    return out, it, float(tr[0])
+
+
+# ---- DWF ----------------------------------------------------------------
+def dwf_struct(L, ls, m5, a5_inv=1.0):
+    d = OraDwf()
+    d.L[:] = [int(v) for v in L]
+    d.ls, d.m5, d.a5_inv = ls, m5, a5_inv
+    return d
+
+
+def dwf_half(d):
+    return 24 * vol(d.L) * d.ls // 2
+
+
+def dwf_dslash(u, inp, mass, cb, dag, d):
+    out = np.zeros(dwf_half(d))
+    lib().ora_dwf_dslash(_p(out), _p(u), _p(inp), mass, cb, dag, ctypes.byref(d))
+    return out
+
+
+def dwf_m(u, inp, mass, d):
+    out, t = np.zeros(dwf_half(d)), np.zeros(dwf_half(d))
+    lib().ora_dwf_m(_p(out), _p(u), _p(inp), mass, ctypes.byref(d), _p(t))
+    return out
+
+
+def dwf_mdag(u, inp, mass, d):
+    out, t = np.zeros(dwf_half(d)), np.zeros(dwf_half(d))
+    lib().ora_dwf_mdag(_p(out), _p(u), _p(inp), mass, ctypes.byref(d), _p(t))
+    return out
+
+
+def dwf_mdagm(u, inp, mass, d):
+    n = dwf_half(d)
+    out, t1, t2 = np.zeros(n), np.zeros(n), np.zeros(n)
+    dp = np.zeros(1)
+    lib().ora_dwf_mdagm(_p(out), _p(u), _p(inp), _p(dp), mass, ctypes.byref(d), _p(t1), _p(t2))
+    return out, float(dp[0])
+
+
+def dwf_invcg(u, src, mass, stop_rsd, max_iter, d, guess=None):
+    sol = np.zeros(dwf_half(d)) if guess is None else guess.copy()
+    tr = np.zeros(1)
+    it = lib().ora_dwf_invcg(_p(sol), _p(src), _p(u), mass, stop_rsd, max_iter, ctypes.byref(d), _p(tr))
+    return sol, it, float(tr[0])
+
+
+def dwf_matinv(u, b, mass, stop_rsd, max_iter, d, guess=None):
+    out = np.zeros(2 * dwf_half(d)) if guess is None else guess.copy()
+    b = b.copy()
+    tr = np.zeros(1)
+    it = lib().ora_dwf_matinv(_p(out), _p(b), _p(u), mass, stop_rsd, max_iter, ctypes.byref(d), _p(tr))
+    return out, it, float(tr[0])
+
+
+# ---- Moebius --------------------------------------------------------------
+def mobius_struct(L, ls, m5, b, c, a5_inv=1.0):
+    m = OraMobius()
+    m.L[:] = [int(v) for v in L]
+    m.ls, m.m5, m.a5_inv, m.b, m.c = ls, m5, a5_inv, b, c
+    return m
+
+
+def mobius_m(u, inp, mass, m):
+    n = dwf_half(m)
+    out, t = np.zeros(n), np.zeros(n)
+    lib().ora_mobius_m(_p(out), _p(u), _p(inp), mass, ctypes.byref(m), _p(t))
+    return out
+
+
+def mobius_mdag(u, inp, mass, m):
+    n = dwf_half(m)
+    out, t, t3 = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().ora_mobius_mdag(_p(out), _p(u), _p(inp), mass, ctypes.byref(m), _p(t), _p(t3))
+    return out
+
+
+def mobius_mdagm(u, inp, mass, m):
+    n = dwf_half(m)
+    out, t1, t2, t3 = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    dp = np.zeros(1)
+    lib().ora_mobius_mdagm(_p(out), _p(u), _p(inp), _p(dp), mass, ctypes.byref(m), _p(t1), _p(t2), _p(t3))
+    return out, float(dp[0])
+
+
+def mobius_m5inv(f, mass, dag, m):
+    out = f.copy()
+    lib().ora_mobius_m5inv(_p(out), mass, dag, ctypes.byref(m))
+    return out
+
+
+def mobius_dslash_5_plus(out, inp, mass, dag, coef, m):
+    out = out.copy()
+    lib().ora_mobius_dslash_5_plus(_p(out), _p(inp), mass, dag, coef, ctypes.byref(m))
+    return out
+
+
+def mobius_dslash_4(u, inp, cb, dag, m, mass):
+    out = np.zeros(dwf_half(m))
+    lib().ora_mobius_dslash_4(_p(out), _p(u), _p(inp), cb, dag, ctypes.byref(m), mass)
+    return out
+
+
+def mobius_invcg(u, src, mass, stop_rsd, max_iter, m, guess=None):
+    sol = np.zeros(dwf_half(m)) if guess is None else guess.copy()
+    tr = np.zeros(1)
+    it = lib().ora_mobius_invcg(_p(sol), _p(src), _p(u), mass, stop_rsd, max_iter, ctypes.byref(m), _p(tr))
+    return sol, it, float(tr[0])
+
+
+# ---- clover ---------------------------------------------------------------
+def clover_mat_chkb(u, cb, inverse, kappa, csw, L):
+    A = np.zeros(72 * vol(L) // 2)
+    lib().ora_clover_mat_chkb(_p(A), _p(u), cb, inverse, kappa, csw, _L(L))
+    return A
+
+
+def clover_mat_mlt(A, inp, nsites):
+    out = np.zeros_like(inp)
+    lib().ora_clover_mat_mlt(_p(out), _p(A), _p(inp), nsites)
+    return out
+
+
+def clover_matpc(u, A_oo, A_ee_inv, inp, kappa, dag, L):
+    n = half(L)
+    out, b0, b1 = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().ora_clover_matpc(_p(out), _p(u), _p(A_oo), _p(A_ee_inv), _p(inp), kappa, dag, _L(L), _p(b0), _p(b1))
+    return out
+
+
+def clover_mdagm(u, A_oo, A_ee_inv, inp, kappa, L):
+    n = half(L)
+    out, t1, t2, t3 = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    dp = np.zeros(1)
+    lib().ora_clover_mdagm(_p(out), _p(u), _p(A_oo), _p(A_ee_inv), _p(inp), _p(dp), kappa, _L(L), _p(t1), _p(t2), _p(t3))
+    return out, float(dp[0])
+
+
+def clover_invcg(u, A_oo, A_ee_inv, src, kappa, stop_rsd, max_iter, L, guess=None):
+    sol = np.zeros(half(L)) if guess is None else guess.copy()
+    tr = np.zeros(1)
+    it = lib().ora_clover_invcg(_p(sol), _p(src), _p(u), _p(A_oo), _p(A_ee_inv), kappa, stop_rsd, max_iter, _L(L), _p(tr))
+    return sol, it, float(tr[0])

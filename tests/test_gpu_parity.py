@@ -1,0 +1,243 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Tolerances: FP64 operators 1e-13 relative (rounding of a
+different summation order only); FP64 CG solutions 1e-10 relative (the
+north-star bound); mixed-precision CG true residual <= stop_rsd, solution
+within 1e-8 of the FP64 oracle."""
+import numpy as np
+import pytest
+
+import cps_public_amd as C
+from cps_public_amd import fields as F
+from tests.helpers import rel, wilson_gauge
+
+pytestmark = pytest.mark.gpu
+
+L4 = [4, 4, 4, 4]
+L8 = [8, 8, 8, 8]
+
+
+@pytest.fixture(scope="module")
+def lat4():
+    lat = C.Lattice(L4, ls=1, bc=(0, 0, 0, 1))
+    u = wilson_gauge(L4, 0.3, 11)
+    lat.import_gauge(u)
+    yield lat, u
+    lat.close()
+
+
+@pytest.mark.parametrize("cb", [0, 1])
+@pytest.mark.parametrize("dag", [0, 1])
+def test_wilson_dslash(oracle, lat4, cb, dag):
+    lat, u = lat4
+    psi = F.gaussian_fermion(lat.vcb, 100 + cb)
+    ref = oracle.wilson_dslash(u, psi, cb, dag, L4)
+    got = lat.wilson_dslash(psi, cb, dag)
+    print("wilson_dslash", cb, dag, rel(got, ref), flush=True)
+    assert rel(got, ref) < 1e-13
+
+
+def test_wilson_dslash_passes_gauge(oracle):
+    # the gauge passed as an argument replaces the resident one (wilson.h semantics)
+    lat = C.Lattice(L4, 1, (0, 0, 0, 1))
+    u = wilson_gauge(L4, 0.3, 12)
+    psi = F.gaussian_fermion(lat.vcb, 5)
+    got = lat.wilson_dslash(psi, 1, 0, u=u)
+    assert rel(got, oracle.wilson_dslash(u, psi, 1, 0, L4)) < 1e-13
+    lat.close()
+
+
+def test_wilson_mdagm(oracle, lat4):
+    lat, u = lat4
+    psi = F.gaussian_fermion(lat.vcb, 7)
+    kappa = 0.12
+    ref, mp_ref = oracle.wilson_mdagm(u, psi, kappa, L4)
+    got, mp = lat.wilson_mdagm(psi, kappa)
+    assert rel(got, ref) < 1e-13
+    assert abs(mp - mp_ref) / mp_ref < 1e-13
+    assert rel(lat.wilson_m(psi, kappa), oracle.wilson_m(u, psi, kappa, L4)) < 1e-13
+    assert rel(lat.wilson_m(psi, kappa, dag=1), oracle.wilson_mdag(u, psi, kappa, L4)) < 1e-13
+
+
+def test_wilson_cg_c1(oracle):
+    """Wilson 8^4, eps 0.3, kappa 0.12, CG to 1e-10 (C1)."""
+    lat = C.Lattice(L8, 1, (0, 0, 0, 1))
+    u = wilson_gauge(L8, 0.3, 2024)
+    lat.import_gauge(u)
+    src = F.gaussian_fermion(lat.vcb, 2025)
+    mass = 1.0 / (2 * 0.12) - 4.0
+    ref, it_ref, tr_ref = oracle.wilson_invcg(u, src, mass, 1e-10, 5000, L8)
+    op = C.make_op(C.OP_WILSON, mass=mass)
+    got, res = lat.invcg(op, src, C.make_cg_arg(1e-10, 5000))
+    print("wilson cg", it_ref, res[0].iterations, tr_ref, res[0].true_res, rel(got, ref), flush=True)
+    assert abs(res[0].iterations - it_ref) <= 1
+    assert res[0].true_res < 1e-10
+    assert rel(got, ref) < 1e-10
+    lat.close()
+
+
+@pytest.fixture(scope="module")
+def dwf4():
+    L = [4, 4, 4, 8]
+    ls = 8
+    lat = C.Lattice(L, ls=ls, bc=(0, 0, 0, 1))
+    u = wilson_gauge(L, 0.2, 31)
+    lat.import_gauge(u)
+    yield lat, u, L, ls
+    lat.close()
+
+
+@pytest.mark.parametrize("cb", [0, 1])
+@pytest.mark.parametrize("dag", [0, 1])
+def test_dwf_dslash(oracle, dwf4, cb, dag):
+    lat, u, L, ls = dwf4
+    d = oracle.dwf_struct(L, ls, 1.8)
+    inp = F.gaussian_fermion(lat.vcb * ls, 40 + cb)
+    ref = oracle.dwf_dslash(u, inp, 0.01, cb, dag, d)
+    got = lat.dwf_dslash(inp, 0.01, cb, dag)
+    print("dwf_dslash", cb, dag, rel(got, ref), flush=True)
+    assert rel(got, ref) < 1e-13
+
+
+def test_dwf_mdagm(oracle, dwf4):
+    lat, u, L, ls = dwf4
+    d = oracle.dwf_struct(L, ls, 1.8)
+    inp = F.gaussian_fermion(lat.vcb * ls, 43)
+    ref, dp_ref = oracle.dwf_mdagm(u, inp, 0.01, d)
+    got, dp = lat.dwf_mdagm(inp, 0.01)
+    assert rel(got, ref) < 1e-13
+    assert abs(dp - dp_ref) / dp_ref < 1e-13
+
+
+def test_dwf_cg(oracle, dwf4):
+    lat, u, L, ls = dwf4
+    d = oracle.dwf_struct(L, ls, 1.8)
+    src = F.gaussian_fermion(lat.vcb * ls, 44)
+    ref, it_ref, tr_ref = oracle.dwf_invcg(u, src, 0.01, 1e-10, 5000, d)
+    op = C.make_op(C.OP_DWF, mass=0.01, m5=1.8)
+    got, res = lat.invcg(op, src, C.make_cg_arg(1e-10, 5000))
+    print("dwf cg", it_ref, res[0].iterations, tr_ref, res[0].true_res, rel(got, ref), flush=True)
+    assert abs(res[0].iterations - it_ref) <= 1
+    assert res[0].true_res < 1e-10
+    assert rel(got, ref) < 1e-10
+
+
+@pytest.mark.parametrize("prec", [C.PREC_MIXED_HALF, C.PREC_MIXED_BF16, C.PREC_MIXED_SINGLE])
+def test_dwf_mixed_cg(oracle, dwf4, prec):
+    lat, u, L, ls = dwf4
+    d = oracle.dwf_struct(L, ls, 1.8)
+    src = F.gaussian_fermion(lat.vcb * ls, 45)
+    ref, it_ref, _ = oracle.dwf_invcg(u, src, 0.01, 1e-12, 5000, d)
+    op = C.make_op(C.OP_DWF, mass=0.01, m5=1.8)
+    got, res = lat.invcg(op, src, C.make_cg_arg(1e-10, 5000, precision=prec, reliable_every=50))
+    print("dwf mixed", prec, it_ref, res[0].iterations, res[0].true_res, res[0].reliable_updates,
+          rel(got, ref), flush=True)
+    assert res[0].status == 1
+    assert res[0].true_res <= 1e-10
+    assert rel(got, ref) < 1e-8
+
+
+def test_dwf_multi_rhs_matches_single(dwf4):
+    lat, u, L, ls = dwf4
+    op = C.make_op(C.OP_DWF, mass=0.01, m5=1.8)
+    n = 24 * lat.vcb * ls
+    srcs = [F.gaussian_fermion(lat.vcb * ls, 60 + k) for k in range(3)]
+    arg = C.make_cg_arg(1e-10, 5000)
+    batched, res = lat.invcg(op, np.concatenate(srcs), arg, nrhs=3)
+    for k in range(3):
+        single, r1 = lat.invcg(op, srcs[k], arg)
+        assert res[k].iterations == r1[0].iterations
+        assert rel(batched[k * n:(k + 1) * n], single) < 1e-13
+
+
+@pytest.fixture(scope="module")
+def mob4():
+    L = [4, 4, 4, 4]
+    ls = 8
+    lat = C.Lattice(L, ls=ls, bc=(0, 0, 0, 1))
+    u = wilson_gauge(L, 0.2, 77)
+    lat.import_gauge(u)
+    yield lat, u, L, ls
+    lat.close()
+
+
+def test_mobius_m_and_mdagm(oracle, mob4):
+    lat, u, L, ls = mob4
+    m = oracle.mobius_struct(L, ls, 1.8, 1.5, 0.5)
+    op = C.make_op(C.OP_MOBIUS, mass=0.005, m5=1.8, b=1.5, c=0.5)
+    inp = F.gaussian_fermion(lat.vcb * ls, 78)
+    assert rel(lat.matpc(op, inp, 0), oracle.mobius_m(u, inp, 0.005, m)) < 1e-13
+    assert rel(lat.matpc(op, inp, 1), oracle.mobius_mdag(u, inp, 0.005, m)) < 1e-13
+    ref, dp_ref = oracle.mobius_mdagm(u, inp, 0.005, m)
+    got, dp = lat.mdagm(op, inp)
+    assert rel(got, ref) < 1e-13
+    assert abs(dp[0] - dp_ref) / dp_ref < 1e-13
+
+
+def test_mobius_cg(oracle, mob4):
+    lat, u, L, ls = mob4
+    m = oracle.mobius_struct(L, ls, 1.8, 1.5, 0.5)
+    op = C.make_op(C.OP_MOBIUS, mass=0.005, m5=1.8, b=1.5, c=0.5)
+    src = F.gaussian_fermion(lat.vcb * ls, 79)
+    ref, it_ref, _ = oracle.mobius_invcg(u, src, 0.005, 1e-10, 5000, m)
+    got, res = lat.invcg(op, src, C.make_cg_arg(1e-10, 5000))
+    print("mobius cg", it_ref, res[0].iterations, res[0].true_res, rel(got, ref), flush=True)
+    assert abs(res[0].iterations - it_ref) <= 1
+    assert rel(got, ref) < 1e-10
+    got2, res2 = lat.invcg(op, src, C.make_cg_arg(1e-10, 5000, precision=C.PREC_MIXED_HALF))
+    assert res2[0].true_res <= 1e-10
+
+
+def test_wilson_matinv(oracle):
+    lat = C.Lattice(L4, 1, (0, 0, 0, 1))
+    u = wilson_gauge(L4, 0.3, 13)
+    lat.import_gauge(u)
+    b = F.gaussian_fermion(2 * lat.vcb, 14)
+    mass = 0.1
+    ref, it_ref, _ = oracle.wilson_matinv(u, b, mass, 1e-12, 500, L4)
+    op = C.make_op(C.OP_WILSON, mass=mass)
+    got, res = lat.matinv(op, b, C.make_cg_arg(1e-12, 500))
+    assert abs(res[0].iterations - it_ref) <= 1
+    assert rel(got, ref) < 1e-10
+    lat.close()
+
+
+def test_dwf_matinv(oracle, dwf4):
+    lat, u, L, ls = dwf4
+    d = oracle.dwf_struct(L, ls, 1.8)
+    b = F.gaussian_fermion(2 * lat.vcb * ls, 15)
+    ref, it_ref, _ = oracle.dwf_matinv(u, b, 0.05, 1e-12, 5000, d)
+    op = C.make_op(C.OP_DWF, mass=0.05, m5=1.8)
+    got, res = lat.matinv(op, b, C.make_cg_arg(1e-12, 5000))
+    assert abs(res[0].iterations - it_ref) <= 1
+    assert rel(got, ref) < 1e-10
+
+
+SLOPPY_TOL = {C.PREC_MIXED_SINGLE: 1e-5, C.PREC_MIXED_HALF: 5e-3, C.PREC_MIXED_BF16: 3e-2}
+
+
+@pytest.mark.parametrize("prec", [C.PREC_MIXED_SINGLE, C.PREC_MIXED_HALF, C.PREC_MIXED_BF16])
+@pytest.mark.parametrize("kind", ["wilson", "dwf", "mobius"])
+def test_sloppy_operator(oracle, prec, kind):
+    """The reduced-precision MdagM the mixed CG iterates with, against the FP64
+    oracle; the tolerance is the storage rounding (FP32 ~1e-7, FP16 spinors
+    2^-11 of the site maximum, BF16 links 2^-8) compounded over four hops."""
+    L = [4, 4, 4, 8]
+    ls = 1 if kind == "wilson" else 8
+    lat = C.Lattice(L, ls, (0, 0, 0, 1))
+    u = wilson_gauge(L, 0.2, 90)
+    lat.import_gauge(u)
+    inp = F.gaussian_fermion(lat.vcb * ls, 91)
+    if kind == "wilson":
+        op = C.make_op(C.OP_WILSON, mass=0.1)
+        ref, dref = oracle.wilson_mdagm(u, inp, 1.0 / (2 * 4.1), L)
+    elif kind == "dwf":
+        op = C.make_op(C.OP_DWF, mass=0.01, m5=1.8)
+        ref, dref = oracle.dwf_mdagm(u, inp, 0.01, oracle.dwf_struct(L, ls, 1.8))
+    else:
+        op = C.make_op(C.OP_MOBIUS, mass=0.01, m5=1.8, b=1.5, c=0.5)
+        ref, dref = oracle.mobius_mdagm(u, inp, 0.01, oracle.mobius_struct(L, ls, 1.8, 1.5, 0.5))
+    got, d = lat.mdagm_prec(op, inp, prec)
+    print("sloppy", kind, prec, rel(got, ref), abs(d[0] - dref) / dref, flush=True)
+    assert rel(got, ref) < SLOPPY_TOL[prec]
+    assert abs(d[0] - dref) / dref < SLOPPY_TOL[prec]
+    lat.close()
