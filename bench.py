@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark: domain-wall CG time-to-solution per right-hand side at true
+residual 1e-10, mixed precision (+ batched right-hand sides) against plain
+FP64 CG, and the HBM bandwidth of the dominant hopping kernel -- on one
+MI355X through libcps_hip.so.
+
+A "step" is one complete CG solve of the workload's right-hand sides with
+sources and gauge resident in HBM (host<->device copies of the vectors are
+outside the kernels but inside the timed call).
+
+Workloads (--config):
+  c2 (default)  Shamir DWF 16^3x32, Ls=16, M5=1.8, mf=0.01, links exp(i 0.2 H),
+                one Gaussian source; mixed = FP32 math, FP16 spinors and links,
+                FP64 reliable updates at least every 100 iterations.
+  c3            Moebius DWF 24^3x64, Ls=16, b=1.5, c=0.5, M5=1.8, mf=0.005,
+                12 point sources (spin x colour) in one batched solve, BF16
+                links, FP16 spinors.
+  c1            Wilson 8^4, kappa 0.12, links exp(i 0.3 H), FP64 only.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import cps_public_amd as C  # noqa: E402
+from cps_public_amd import fields as F  # noqa: E402
+from cps_public_amd import layout as LY  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    "c1": dict(name="wilson_8^4_kappa0.12", L=[8, 8, 8, 8], ls=1, kind=C.OP_WILSON, eps=0.3,
+               mass=1.0 / (2 * 0.12) - 4.0, nrhs=1, src="gauss", mixed=None, seed=101),
+    "c2": dict(name="shamir_dwf_16^3x32_ls16_mf0.01", L=[16, 16, 16, 32], ls=16, kind=C.OP_DWF, eps=0.2,
+               mass=0.01, m5=1.8, nrhs=1, src="gauss", mixed=C.PREC_MIXED_HALF, seed=202),
+    "c3": dict(name="mobius_dwf_24^3x64_ls16_b1.5_c0.5_mf0.005_12rhs", L=[24, 24, 24, 64], ls=16,
+               kind=C.OP_MOBIUS, eps=0.2, mass=0.005, m5=1.8, b=1.5, c=0.5, nrhs=12, src="point",
+               mixed=C.PREC_MIXED_BF16, seed=303),
+}
+
+
+def make_inputs(cfg):
+    L, ls = cfg["L"], cfg["ls"]
+    u = F.gauge_field(L, cfg["eps"], cfg["seed"])
+    u_w = LY.gauge_canon_to_wilson(LY.bond_cond(u, L, [0, 0, 0, 1]), L)
+    vcb = int(np.prod(L)) // 2
+    per = 24 * vcb * ls
+    if cfg["src"] == "gauss":
+        src = np.concatenate([F.gaussian_fermion(vcb * ls, cfg["seed"] + 1 + k) for k in range(cfg["nrhs"])])
+    else:
+        # point sources delta(x - (1,0,0,0)) delta_{s,0} e_{spin,colour}: (1,0,0,0)
+        # is an odd site, so the source lies on the checkerboard CG works on
+        src = np.zeros(per * cfg["nrhs"])
+        site_cb = 0  # x/2 + Lx/2*(y + ...) for x = 1
+        for k in range(cfg["nrhs"]):
+            spin, colour = divmod(k % 12, 3)
+            src[k * per + 24 * site_cb + 2 * (colour + 3 * spin)] = 1.0
+    return u_w, src
+
+
+def op_for(cfg):
+    return C.make_op(cfg["kind"], mass=cfg["mass"], m5=cfg.get("m5", 1.8), b=cfg.get("b", 1.0),
+                     c=cfg.get("c", 0.0))
+
+
+def time_solves(lat, op, src, arg, nrhs, warmup, steps):
+    res = None
+    for _ in range(warmup):
+        _, res = lat.invcg(op, src, arg, nrhs=nrhs)
+    t = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        _, res = lat.invcg(op, src, arg, nrhs=nrhs)
+        t.append(time.perf_counter() - t0)
+        print(f"# solve: {t[-1]:.4f} s, iterations {[r.iterations for r in res]}, "
+              f"true_res {max(r.true_res for r in res):.3e}", file=sys.stderr, flush=True)
+    return t, res
+
+
+def cpu_baseline(cfg, u_w, src, gpu_iters):
+    """The CPU restatement (oracle/) of the CPS noarch FP64 CG, one thread,
+    on the same gauge and source, for a bounded number of iterations; the
+    per-iteration time is scaled to the GPU FP64 iteration count."""
+    from oracle import oracle as O
+    O.build()
+    L, ls = cfg["L"], cfg["ls"]
+    per = 24 * (int(np.prod(L)) // 2) * ls
+    s0 = np.ascontiguousarray(src[:per])
+    n_it = 3 if ls > 1 else 50
+    t0 = time.perf_counter()
+    if cfg["kind"] == C.OP_DWF:
+        d = O.dwf_struct(L, ls, cfg["m5"])
+        _, it, _ = O.dwf_invcg(u_w, s0, cfg["mass"], 1e-30, n_it + 1, d)
+    elif cfg["kind"] == C.OP_MOBIUS:
+        m = O.mobius_struct(L, ls, cfg["m5"], cfg["b"], cfg["c"])
+        _, it, _ = O.mobius_invcg(u_w, s0, cfg["mass"], 1e-30, n_it + 1, m)
+    else:
+        _, it, _ = O.wilson_invcg(u_w, s0, cfg["mass"], 1e-30, n_it + 1, L)
+    el = time.perf_counter() - t0
+    # each oracle CG runs n_it iterations plus two extra MdagM (initial and true residual)
+    per_iter = el / (n_it + 2)
+    return {"value": per_iter * gpu_iters, "unit": "s/RHS", "cores": 1, "kind": "port",
+            "sample": f"{n_it} CG iterations (+2 MdagM) of the oracle FP64 CG on rhs 0 of the same workload, "
+                      f"{el:.1f} s; per-iteration time x {gpu_iters} iterations (the GPU FP64 count)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
+    ap.add_argument("--hop-reps", type=int, default=50)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only launch the hopping kernels (mixed and FP64) and the reference copy kernel, "
+                         "for hardware-counter runs")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {a.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    cfg = CONFIGS[a.config]
+    u_w, src = make_inputs(cfg)
+    lat = C.Lattice(cfg["L"], cfg["ls"], (0, 0, 0, 1))
+    lat.import_gauge(u_w)
+    op = op_for(cfg)
+    nrhs = cfg["nrhs"]
+
+    mixed = cfg["mixed"]
+    prec_main = mixed if mixed is not None else C.PREC_DOUBLE
+    if a.kernels_only:
+        sec, byt = lat.bench_hop(cfg["kind"], prec_main, nrhs, a.hop_reps)
+        sec64, byt64 = lat.bench_hop(cfg["kind"], C.PREC_DOUBLE, nrhs, a.hop_reps)
+        ncopy = (int(np.prod(cfg["L"])) // 2) * cfg["ls"] * nrhs
+        secc = lat.bench_copy(ncopy, a.hop_reps)
+        print(json.dumps({"hop_mixed": {"s": sec, "bytes": byt}, "hop_fp64": {"s": sec64, "bytes": byt64},
+                          "copy": {"s": secc, "bytes_read": 192 * ncopy, "bytes_written": 192 * ncopy}}))
+        lat.close()
+        return
+    arg_main = C.make_cg_arg(1e-10, 20000, precision=prec_main, reliable_every=100)
+    t_main, res_main = time_solves(lat, op, src, arg_main, nrhs, a.warmup, a.steps)
+    ms = 1e3 * float(np.mean(t_main))
+    s_per_rhs = float(np.mean(t_main)) / nrhs
+
+    fp64 = None
+    if mixed is not None and not a.no_fp64:
+        arg64 = C.make_cg_arg(1e-10, 20000, precision=C.PREC_DOUBLE)
+        t64, res64 = time_solves(lat, op, src, arg64, nrhs, 0, 1)
+        fp64 = {"s_per_rhs": float(np.mean(t64)) / nrhs, "iterations": [r.iterations for r in res64],
+                "true_res_max": max(r.true_res for r in res64)}
+    gpu_fp64_iters = (fp64["iterations"][0] if fp64 else res_main[0].iterations)
+
+    # dominant kernel: the hopping kernel of the headline path, timed with HIP
+    # events on its launch stream
+    hk = cfg["kind"] if cfg["kind"] != C.OP_MOBIUS else C.OP_MOBIUS
+    sec, byt = lat.bench_hop(hk, prec_main, nrhs, a.hop_reps)
+    achieved = byt / sec / 1e9
+    sec64, byt64 = lat.bench_hop(hk, C.PREC_DOUBLE, nrhs, a.hop_reps)
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{a.config}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(cfg, u_w, src, gpu_fp64_iters)
+
+    line = {
+        "metric": "DWF CG time-to-solution/RHS @1e-10 & dslash HBM GB/s: mixed+batched vs FP64",
+        "value": s_per_rhs,
+        "unit": "s/RHS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 math / f16 spinors, f64 reliable updates" if mixed is not None else "f64",
+        "data": "synthetic (seeded SU(3) links exp(i eps H), Philox)",
+        "config": {"workload": cfg["name"], "nrhs": nrhs, "stop_rsd": 1e-10,
+                   "precision": {0: "fp64", 1: "mixed_fp16", 2: "mixed_bf16links", 3: "mixed_fp32"}[prec_main]},
+        "iterations": [r.iterations for r in res_main],
+        "true_res_max": max(r.true_res for r in res_main),
+        "reliable_updates": [r.reliable_updates for r in res_main],
+        "fp64": fp64,
+        "speedup_vs_fp64": (fp64["s_per_rhs"] / s_per_rhs) if fp64 else None,
+        "dslash": {"kernel": "k_hop (5-D DWF)" if cfg["kind"] == C.OP_DWF else "k_hop (4-D, batched)",
+                   "seconds_per_launch": sec, "algorithmic_bytes": byt, "GBps": achieved,
+                   "fp64_GBps": byt64 / sec64 / 1e9, "fp64_seconds_per_launch": sec64},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    lat.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -82,6 +82,7 @@ def _declare(l):
         "cps_hip_invcg": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_matinv": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_bench_hop": [vp, i, i, i, i, P(d), P(d)],
+        "cps_hip_bench_copy": [vp, ctypes.c_longlong, i, P(d)],
         "cps_hip_device_count": [],
     }
     for name, args in sig.items():
@@ -231,6 +232,12 @@ class Lattice:
         _check(lib().cps_hip_bench_hop(self.h, kind, precision, nrhs, reps, ctypes.byref(s), ctypes.byref(b)),
                "cps_hip_bench_hop")
         return s.value, b.value
+
+
+    def bench_copy(self, n_elem, reps=20):
+        s = ctypes.c_double()
+        _check(lib().cps_hip_bench_copy(self.h, n_elem, reps, ctypes.byref(s)), "cps_hip_bench_copy")
+        return s.value
 
 
 def device_count():

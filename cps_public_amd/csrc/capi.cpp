@@ -409,6 +409,29 @@ int cps_hip_matinv(cps_hip_lattice* lat, const cps_hip_op* op, double* out, cons
 }
 
 // ---- measurement -------------------------------------------------------------
+int cps_hip_bench_copy(cps_hip_lattice* lat, long long n_elem, int reps, double* seconds) {
+  if (check_lat(lat) || n_elem < 1 || reps < 1) return CPS_HIP_ERR_ARG;
+  Lattice& L = lat->L;
+  Field& a = L.field(S_BENCH_IN, STORE_D, n_elem);
+  Field& b = L.field(S_BENCH_OUT, STORE_D, n_elem);
+  if (L.last_rc) return L.last_rc;
+  CPSHIP_CHECK(fill_spinor(STORE_D, a.ref(), n_elem, 3u, L.stream));
+  CPSHIP_CHECK(convert_spinor(STORE_D, a.ref(), STORE_D, b.ref(), n_elem, L.stream));
+  hipEvent_t e0, e1;
+  CPSHIP_CHECK(hipEventCreate(&e0));
+  CPSHIP_CHECK(hipEventCreate(&e1));
+  CPSHIP_CHECK(hipEventRecord(e0, L.stream));
+  for (int r = 0; r < reps; r++) CPSHIP_CHECK(convert_spinor(STORE_D, a.ref(), STORE_D, b.ref(), n_elem, L.stream));
+  CPSHIP_CHECK(hipEventRecord(e1, L.stream));
+  CPSHIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  CPSHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (seconds) *seconds = ms * 1e-3 / reps;
+  return 0;
+}
+
 int cps_hip_bench_hop(cps_hip_lattice* lat, int kind, int precision, int nrhs, int reps, double* seconds,
                       double* bytes) {
   if (check_lat(lat) || nrhs < 1 || reps < 1) return CPS_HIP_ERR_ARG;

@@ -172,6 +172,11 @@ int cps_hip_matinv(cps_hip_lattice *lat, const cps_hip_op *op, double *out, cons
 int cps_hip_bench_hop(cps_hip_lattice *lat, int kind, int precision, int nrhs, int reps, double *seconds,
                       double *bytes);
 
+/* Time `reps` launches of a plain device copy kernel (FP64 field to FP64
+ * field) of n_elem spinors (192 bytes read + 192 bytes written each): the
+ * known-size reference that scales hardware traffic counters. */
+int cps_hip_bench_copy(cps_hip_lattice *lat, long long n_elem, int reps, double *seconds);
+
 int cps_hip_device_count(void);
 const char *cps_hip_version(void);
 
