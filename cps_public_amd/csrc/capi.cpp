@@ -240,14 +240,9 @@ int cps_hip_matpc_dag_matpc_prec(cps_hip_lattice* lat, const cps_hip_op* op, int
   if (check_lat(lat) || !op || !out || !in || nrhs < 1) return CPS_HIP_ERR_ARG;
   Lattice& L = lat->L;
   if (!L.has_gauge) return CPS_HIP_ERR_NOGAUGE;
+  if (precision < CPS_HIP_PREC_DOUBLE || precision > CPS_HIP_PREC_MIXED_SINGLE) return CPS_HIP_ERR_ARG;
   int hprec, st;
-  switch (precision) {
-    case CPS_HIP_PREC_DOUBLE: hprec = PREC_D_D18; st = STORE_D; break;
-    case CPS_HIP_PREC_MIXED_SINGLE: hprec = PREC_F_F12; st = STORE_F; break;
-    case CPS_HIP_PREC_MIXED_BF16: hprec = PREC_H_B12; st = STORE_H; break;
-    case CPS_HIP_PREC_MIXED_HALF: hprec = PREC_H_H12; st = STORE_H; break;
-    default: return CPS_HIP_ERR_ARG;
-  }
+  precision_formats(precision, op, L.ls, nrhs, &st, &hprec);
   OpLayout o;
   int rc = op_layout(op, L.ls, &o, L.g);
   if (rc) return rc;
@@ -446,13 +441,9 @@ int cps_hip_bench_hop(cps_hip_lattice* lat, int kind, int precision, int nrhs, i
   OpLayout o;
   int rc = op_layout(&op, L.ls, &o, L.g);
   if (rc) return rc;
+  if (precision < CPS_HIP_PREC_DOUBLE || precision > CPS_HIP_PREC_MIXED_SINGLE) return CPS_HIP_ERR_ARG;
   int hprec, st;
-  switch (precision) {
-    case CPS_HIP_PREC_DOUBLE: hprec = PREC_D_D18; st = STORE_D; break;
-    case CPS_HIP_PREC_MIXED_SINGLE: hprec = PREC_F_F12; st = STORE_F; break;
-    case CPS_HIP_PREC_MIXED_BF16: hprec = PREC_H_B12; st = STORE_H; break;
-    default: hprec = PREC_H_H12; st = STORE_H; break;
-  }
+  precision_formats(precision, &op, L.ls, nrhs, &st, &hprec);
   const int64_t n = o.per_rhs * nrhs;
   Field& fi = L.field(S_BENCH_IN, st, n);
   Field& fo = L.field(S_BENCH_OUT, st, n);
@@ -480,7 +471,7 @@ int cps_hip_bench_hop(cps_hip_lattice* lat, int kind, int precision, int nrhs, i
     // compulsory traffic: every link once (4V links), each spinor read and
     // written once per rhs (compulsory-traffic model)
     const double V = 2.0 * L.g.Vcb;
-    const double lb = hprec == PREC_D_D18 ? 18 * 8 : hprec == PREC_F_F12 ? 12 * 4 : 12 * 2;
+    const double lb = (double)link_bytes(link_storage_of(hprec));
     const double sb = st == STORE_D ? 24 * 8 : st == STORE_F ? 24 * 4 : 24 * 2 + 4;
     *bytes = 4 * V * lb + (double)n * 2 * sb;
   }

@@ -12,9 +12,18 @@ enum HopPrec {
   PREC_F_F12 = 1,   // FP32 spinors, FP32 12-real links, FP32 math
   PREC_H_H12 = 2,   // FP16 spinors + norm, FP16 12-real links, FP32 math
   PREC_H_B12 = 3,   // FP16 spinors + norm, BF16 12-real links, FP32 math
+  PREC_H_H18 = 4,   // FP16 spinors + norm, FP16 18-real links
+  PREC_H_B18 = 5,   // FP16 spinors + norm, BF16 18-real links
+};
+
+// launch variants chosen by the tuner
+enum HopFlags {
+  HOP_NT_STORE = 1,       // streaming (non-temporal) stores of the output
+  HOP_NO_XCD_REMAP = 2,   // keep the hardware block order
 };
 
 struct HopParams {
+  int flags;         // HopFlags
   Geom geom;
   int kind;          // HopKind
   int prec;          // HopPrec
@@ -36,6 +45,18 @@ struct HopParams {
 };
 
 hipError_t launch_hop(const HopParams& P, hipStream_t st);
+
+// link storage read by a hop precision
+inline int link_storage_of(int hprec) {
+  switch (hprec) {
+    case PREC_D_D18: return 0;   // LINK_D18
+    case PREC_F_F12: return 1;   // LINK_F12
+    case PREC_H_H12: return 2;   // LINK_H12
+    case PREC_H_B12: return 3;   // LINK_B12
+    case PREC_H_H18: return 4;   // LINK_H18
+    default: return 5;           // LINK_B18
+  }
+}
 
 // block size dividing n (a power of two, at most 256)
 inline int pick_block(long long n) {

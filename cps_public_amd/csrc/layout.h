@@ -5,7 +5,7 @@
 namespace cpship {
 
 enum SpinStorage { STORE_D = 0, STORE_F = 1, STORE_H = 2 };
-enum LinkStorage { LINK_D18 = 0, LINK_F12 = 1, LINK_H12 = 2, LINK_B12 = 3 };
+enum LinkStorage { LINK_D18 = 0, LINK_F12 = 1, LINK_H12 = 2, LINK_B12 = 3, LINK_H18 = 4, LINK_B18 = 5, NLINK = 6 };
 enum SpinMap { MAP_4D = 0, MAP_DWF5 = 1, MAP_4DPREC = 2 };
 
 hipError_t pack_spinor(int storage, const double* aos, const FieldRef& out, int map, int Vcb, int ls, int cb5,
@@ -18,10 +18,10 @@ hipError_t pack_gauge(int link_storage, const double* aos, const GaugeRef& out, 
 inline size_t spin_elem_bytes(int storage) {
   return storage == STORE_D ? 16 : storage == STORE_F ? 8 : 4;
 }
-inline size_t link_elem_bytes(int ls_) {
-  return ls_ == LINK_D18 ? 16 : ls_ == LINK_F12 ? 8 : 4;
+// bytes of one stored link (18 FP64 reals, or 12 reals of FP32 / FP16 / BF16)
+inline size_t link_bytes(int ls_) {
+  return ls_ == LINK_D18 ? 144 : ls_ == LINK_F12 ? 48 : (ls_ == LINK_H18 || ls_ == LINK_B18) ? 36 : 24;
 }
-inline int link_planes(int ls_) { return ls_ == LINK_D18 ? 9 : 6; }
 
 }  // namespace cpship
 

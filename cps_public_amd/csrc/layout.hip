@@ -81,15 +81,9 @@ __global__ void k_convert(FieldRef in, FieldRef out, int64_t n) {
   }
 }
 
-template <typename E> __device__ __forceinline__ E make_link_elem(double re, double im);
-template <> __device__ __forceinline__ double2 make_link_elem<double2>(double re, double im) { return make_double2(re, im); }
-template <> __device__ __forceinline__ float2 make_link_elem<float2>(double re, double im) { return make_float2(float(re), float(im)); }
-template <> __device__ __forceinline__ __half2 make_link_elem<__half2>(double re, double im) { return __floats2half2_rn(float(re), float(im)); }
-template <> __device__ __forceinline__ __hip_bfloat162 make_link_elem<__hip_bfloat162>(double re, double im) {
-  __hip_bfloat162 b;
-  b.x = __float2bfloat16(float(re));
-  b.y = __float2bfloat16(float(im));
-  return b;
+__device__ __forceinline__ uint32_t pack_bf16(double re, double im) {
+  __hip_bfloat16 a = __float2bfloat16(float(re)), b = __float2bfloat16(float(im));
+  return (uint32_t)(*reinterpret_cast<uint16_t*>(&a)) | ((uint32_t)(*reinterpret_cast<uint16_t*>(&b)) << 16);
 }
 
 // one thread per (parity, site_cb, mu)
@@ -103,14 +97,37 @@ __global__ void k_pack_gauge(const double* __restrict__ aos, GaugeRef out, int V
   const int parity = (int)(ps / Vcb);
   const int site = (int)(ps - (int64_t)parity * Vcb);
   const double* src = aos + 72 * ps + 18 * mu;
-  using E = typename LinkTraits<LK>::elem;
-  E* dst = reinterpret_cast<E*>(const_cast<void*>(out.data)) + (int64_t)(parity * 4 + mu) * Vcb + site;
-  const int np = LinkTraits<LK>::nplanes;
-  for (int row = 0; row < np / 3; row++)
-    for (int col = 0; col < 3; col++) {
-      const int w = 2 * (row + 3 * col);   // WILSON order: U(row,col) at 2*(row+3*col)
-      dst[(3 * row + col) * out.plane] = make_link_elem<E>(src[w], src[w + 1]);
-    }
+  const int64_t idx = (int64_t)(parity * 4 + mu) * Vcb + site;
+  // element c = 3*row + col; WILSON order holds U(row,col) at 2*(row+3*col)
+  double re[9], im[9];
+  for (int c = 0; c < 9; c++) {
+    const int w = 2 * ((c / 3) + 3 * (c % 3));
+    re[c] = src[w];
+    im[c] = src[w + 1];
+  }
+  void* base = const_cast<void*>(out.data);
+  if constexpr (std::is_same<LK, LinkD18>::value) {
+    double2* p = reinterpret_cast<double2*>(base) + idx;
+    for (int c = 0; c < 9; c++) p[c * out.plane] = make_double2(re[c], im[c]);
+  } else if constexpr (std::is_same<LK, LinkF12>::value) {
+    float4* p = reinterpret_cast<float4*>(base) + idx;
+    for (int q = 0; q < 3; q++)
+      p[q * out.plane] = make_float4(float(re[2 * q]), float(im[2 * q]), float(re[2 * q + 1]), float(im[2 * q + 1]));
+  } else if constexpr (std::is_same<LK, LinkH18>::value || std::is_same<LK, LinkB18>::value) {
+    uint32_t w[9];
+    for (int c = 0; c < 9; c++)
+      w[c] = std::is_same<LK, LinkH18>::value ? f2h(float(re[c]), float(im[c])) : pack_bf16(re[c], im[c]);
+    uint4* a = reinterpret_cast<uint4*>(base);
+    a[idx] = make_uint4(w[0], w[1], w[2], w[3]);
+    a[out.plane + idx] = make_uint4(w[4], w[5], w[6], w[7]);
+    reinterpret_cast<uint32_t*>(a + 2 * out.plane)[idx] = w[8];
+  } else {
+    uint32_t w[6];
+    for (int c = 0; c < 6; c++)
+      w[c] = std::is_same<LK, LinkH12>::value ? f2h(float(re[c]), float(im[c])) : pack_bf16(re[c], im[c]);
+    reinterpret_cast<uint4*>(base)[idx] = make_uint4(w[0], w[1], w[2], w[3]);
+    reinterpret_cast<uint2*>(reinterpret_cast<uint4*>(base) + out.plane)[idx] = make_uint2(w[4], w[5]);
+  }
 }
 
 static inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -165,6 +182,8 @@ hipError_t pack_gauge(int link_storage, const double* aos, const GaugeRef& out, 
     case LINK_F12: hipLaunchKernelGGL(k_pack_gauge<LinkF12>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, Vcb); break;
     case LINK_H12: hipLaunchKernelGGL(k_pack_gauge<LinkH12>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, Vcb); break;
     case LINK_B12: hipLaunchKernelGGL(k_pack_gauge<LinkB12>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, Vcb); break;
+    case LINK_H18: hipLaunchKernelGGL(k_pack_gauge<LinkH18>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, Vcb); break;
+    case LINK_B18: hipLaunchKernelGGL(k_pack_gauge<LinkB18>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, Vcb); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

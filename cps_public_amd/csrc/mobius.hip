@@ -283,12 +283,11 @@ static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, 
   P.in = in;
   P.out = out;
   P.X = null_field();
-  const int lstore = hprec == PREC_D_D18 ? LINK_D18 : hprec == PREC_F_F12 ? LINK_F12 : hprec == PREC_H_H12 ? LINK_H12 : LINK_B12;
+  const int lstore = link_storage_of(hprec);
   int rc = L.ensure_gauge(lstore);
   if (rc) return rc;
   P.gauge = L.gauge_ref(lstore);
-  CPSHIP_CHECK(launch_hop(P, L.stream));
-  return 0;
+  return tuned_launch(L, P);
 }
 
 static S5Params s5_base(Lattice& L, const cps_hip_op* op, int dag, int nrhs) {

@@ -1,5 +1,6 @@
 // Host-side objects behind the C ABI.
 #pragma once
+#include <map>
 #include <vector>
 
 #include "../../include/cps_hip.h"
@@ -42,13 +43,29 @@ struct OpLayout {
   int64_t per_rhs;   // elements of one rhs
 };
 
+struct HopTuneKey {
+  int kind, prec, dag, nb, nsites, nrhs, partial, hasX;
+  bool operator<(const HopTuneKey& o) const {
+    const int a[8] = {kind, prec, dag, nb, nsites, nrhs, partial, hasX};
+    const int b[8] = {o.kind, o.prec, o.dag, o.nb, o.nsites, o.nrhs, o.partial, o.hasX};
+    for (int k = 0; k < 8; k++)
+      if (a[k] != b[k]) return a[k] < b[k];
+    return false;
+  }
+};
+struct HopTuneVal {
+  int block, flags;
+  double ms;
+};
+
 struct Lattice {
   Geom g;
+  std::map<HopTuneKey, HopTuneVal> hop_tune;
   int ls = 1;
   hipStream_t stream = nullptr;
   DevBuf gauge_aos;
-  DevBuf gauge[4];
-  bool gauge_valid[4];
+  DevBuf gauge[NLINK];
+  bool gauge_valid[NLINK];
   bool has_gauge = false;
   long long gauge_generation = 0;
   std::vector<Field> scratch;
@@ -86,6 +103,8 @@ double op_kappa(const cps_hip_op* op);
 int hop(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
         double scale, const FieldRef& X, double xcoef, double* partial, int nrhs);
 int hop_blocks_per_rhs(Lattice& L, const cps_hip_op* op);
+int tuned_launch(Lattice& L, HopParams& P);
+void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp_store, int* hprec);
 int apply_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, const FieldRef& t,
             int dag, double* partial, int nrhs);
 int apply_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,

@@ -57,7 +57,7 @@ FieldRef null_field() {
 
 // ---------------------------------------------------------------------------
 Lattice::Lattice() {
-  for (int k = 0; k < 4; k++) gauge_valid[k] = false;
+  for (int k = 0; k < NLINK; k++) gauge_valid[k] = false;
   // every slot exists up front: references handed out by field() stay valid
   scratch.resize(S_NSLOTS);
 }
@@ -86,7 +86,7 @@ int Lattice::init(const int node_sites[4], int ls_, const int bc[4]) {
 
 long long Lattice::device_bytes() const {
   long long b = (long long)gauge_aos.bytes + partial.bytes + scal.bytes + io.bytes + io2.bytes;
-  for (int k = 0; k < 4; k++) b += gauge[k].bytes;
+  for (int k = 0; k < NLINK; k++) b += gauge[k].bytes;
   for (auto& f : scratch) b += f.buf.bytes + f.nrm.bytes;
   b += clover.bytes;
   return b;
@@ -97,7 +97,7 @@ int Lattice::import_gauge(const double* u_host) {
   int rc = gauge_aos.ensure(n * sizeof(double));
   if (rc) return rc;
   CPSHIP_CHECK(hipMemcpyAsync(gauge_aos.p, u_host, n * sizeof(double), hipMemcpyHostToDevice, stream));
-  for (int k = 0; k < 4; k++) gauge_valid[k] = false;
+  for (int k = 0; k < NLINK; k++) gauge_valid[k] = false;
   rc = ensure_gauge(LINK_D18);
   if (rc) return rc;
   CPSHIP_CHECK(hipStreamSynchronize(stream));
@@ -108,9 +108,8 @@ int Lattice::import_gauge(const double* u_host) {
 
 int Lattice::ensure_gauge(int link_storage) {
   if (gauge_valid[link_storage]) return 0;
-  const size_t planes = link_planes(link_storage);
   const int64_t plane = (int64_t)8 * g.Vcb;
-  int rc = gauge[link_storage].ensure(planes * plane * link_elem_bytes(link_storage));
+  int rc = gauge[link_storage].ensure((size_t)plane * link_bytes(link_storage));
   if (rc) return rc;
   GaugeRef gr;
   gr.data = gauge[link_storage].p;
@@ -210,11 +209,74 @@ int hop(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const 
   P.in = in;
   P.out = out;
   P.X = X;
-  int lstore = hprec == PREC_D_D18 ? LINK_D18 : hprec == PREC_F_F12 ? LINK_F12 : hprec == PREC_H_H12 ? LINK_H12 : LINK_B12;
+  int lstore = link_storage_of(hprec);
   rc = L.ensure_gauge(lstore);
   if (rc) return rc;
   P.gauge = L.gauge_ref(lstore);
   P.partial = partial;
+  rc = tuned_launch(L, P);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: the first launch of each hop shape (kind, precision, dag, batch,
+// volume, rhs count, epilogue) times every candidate block size and launch
+// variant on the real operands and keeps the fastest for the lattice
+// handle's lifetime.  Launches that write reduction partials keep the
+// default block size, because the reduction reads one partial per block.
+// CPS_HIP_TUNE=0 switches tuning off (default block, no variants).
+
+static bool tuning_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("CPS_HIP_TUNE");
+    v = (e && *e == '0') ? 0 : 1;
+  }
+  return v == 1;
+}
+
+int tuned_launch(Lattice& L, HopParams& P) {
+  const long long per_rhs = (long long)P.nsites_out * P.nb;
+  const int def_block = pick_block(per_rhs);
+  HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr};
+  auto it = L.hop_tune.find(key);
+  if (it == L.hop_tune.end()) {
+    HopTuneVal best{def_block, 0, 0.0};
+    if (tuning_enabled()) {
+      std::vector<int> blocks;
+      if (P.partial) blocks.push_back(def_block);
+      else
+        for (int b : {64, 128, 256})
+          if (b <= def_block && per_rhs % b == 0) blocks.push_back(b);
+      hipEvent_t e0, e1;
+      CPSHIP_CHECK(hipEventCreate(&e0));
+      CPSHIP_CHECK(hipEventCreate(&e1));
+      float best_ms = 1e30f;
+      for (int b : blocks)
+        for (int f = 0; f < 4; f++) {
+          P.block = b;
+          P.flags = f;
+          CPSHIP_CHECK(launch_hop(P, L.stream));   // warm-up
+          CPSHIP_CHECK(hipEventRecord(e0, L.stream));
+          for (int r = 0; r < 3; r++) CPSHIP_CHECK(launch_hop(P, L.stream));
+          CPSHIP_CHECK(hipEventRecord(e1, L.stream));
+          CPSHIP_CHECK(hipEventSynchronize(e1));
+          float ms = 0;
+          CPSHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best_ms) {
+            best_ms = ms;
+            best.block = b;
+            best.flags = f;
+          }
+        }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      best.ms = best_ms / 3;
+    }
+    it = L.hop_tune.emplace(key, best).first;
+  }
+  P.block = it->second.block;
+  P.flags = it->second.flags;
   CPSHIP_CHECK(launch_hop(P, L.stream));
   return 0;
 }
@@ -436,12 +498,26 @@ int cg_double_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int
 //   rs = r, xs = 0;
 // a rhs stops only when that FP64 residual meets |r|^2 <= stop_rsd^2 |src|^2.
 
-static void mixed_formats(int prec, int* sp_store, int* hprec) {
+// Storage / arithmetic of a solver precision mode for an operator.  Links are
+// stored 18-real when a site carries at least 4 spinors (DWF slots, Moebius
+// slices, right-hand sides): the links are then a small share of the traffic
+// and skipping the third-row reconstruction saves arithmetic; otherwise the
+// 12-real form saves bandwidth.
+void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp_store, int* hprec) {
+  int per_site = nrhs;
+  if (op->kind == CPS_HIP_OP_DWF) per_site *= ls / 2;
+  if (op->kind == CPS_HIP_OP_MOBIUS) per_site *= ls;
+  const bool full = per_site >= 4;
   switch (prec) {
+    case CPS_HIP_PREC_DOUBLE: *sp_store = STORE_D; *hprec = PREC_D_D18; break;
     case CPS_HIP_PREC_MIXED_SINGLE: *sp_store = STORE_F; *hprec = PREC_F_F12; break;
-    case CPS_HIP_PREC_MIXED_BF16: *sp_store = STORE_H; *hprec = PREC_H_B12; break;
-    default: *sp_store = STORE_H; *hprec = PREC_H_H12; break;
+    case CPS_HIP_PREC_MIXED_BF16: *sp_store = STORE_H; *hprec = full ? PREC_H_B18 : PREC_H_B12; break;
+    default: *sp_store = STORE_H; *hprec = full ? PREC_H_H18 : PREC_H_H12; break;
   }
+}
+
+static void mixed_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp_store, int* hprec) {
+  precision_formats(prec, op, ls, nrhs, sp_store, hprec);
 }
 
 // After a reliable update (rr_true = FP64 |src - MdagM x|^2 per rhs):
@@ -483,7 +559,7 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   int rc = op_layout(op, L.ls, &o, L.g);
   if (rc) return rc;
   int sps, hprec;
-  mixed_formats(arg->precision, &sps, &hprec);
+  mixed_formats(arg->precision, op, L.ls, nrhs, &sps, &hprec);
   const int64_t n = o.per_rhs * nrhs;
   Field& r = L.field(S_R, STORE_D, n);
   Field& ap = L.field(S_AP, STORE_D, n);
