@@ -80,10 +80,14 @@ def _declare(l):
         "cps_hip_clover_build": [vp, vp, i, d],
         "cps_hip_clover_export": [vp, i, i, vp],
         "cps_hip_invcg": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
+        "cps_hip_invcg_device": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_matinv": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_bench_hop": [vp, i, i, i, i, P(d), P(d)],
         "cps_hip_bench_copy": [vp, ctypes.c_longlong, i, P(d)],
         "cps_hip_device_count": [],
+        "cps_hip_device_alloc": [P(vp), ctypes.c_size_t],
+        "cps_hip_device_free": [vp],
+        "cps_hip_memcpy": [vp, vp, ctypes.c_size_t, i],
     }
     for name, args in sig.items():
         f = getattr(l, name)
@@ -220,6 +224,19 @@ class Lattice:
         _check(rc, "cps_hip_invcg")
         return sol, [res[k] for k in range(nrhs)]
 
+    def invcg_device(self, op, sol, src, arg, nrhs=1):
+        """InvCg on device-resident FP64 vectors in CPS layout: sol and src are
+        DeviceBuffer objects of nrhs * f_size_cb reals; sol holds the initial
+        guess and receives the solution in place."""
+        n = self._nreal(op) * nrhs * 8
+        for b, name in ((sol, "sol"), (src, "src")):
+            if not isinstance(b, DeviceBuffer) or b.nbytes != n:
+                raise ValueError(f"invcg_device: {name} must be a DeviceBuffer of {n} bytes")
+        res = (CgResult * nrhs)()
+        rc = lib().cps_hip_invcg_device(self.h, ctypes.byref(op), sol.ptr, src.ptr, nrhs, ctypes.byref(arg), res)
+        _check(rc, "cps_hip_invcg_device")
+        return [res[k] for k in range(nrhs)]
+
     def matinv(self, op, b, arg, guess=None, nrhs=1):
         out = np.zeros(2 * self._nreal(op) * nrhs) if guess is None else np.ascontiguousarray(guess, dtype=np.float64).copy()
         res = (CgResult * nrhs)()
@@ -238,6 +255,48 @@ class Lattice:
         s = ctypes.c_double()
         _check(lib().cps_hip_bench_copy(self.h, n_elem, reps, ctypes.byref(s)), "cps_hip_bench_copy")
         return s.value
+
+
+class DeviceBuffer:
+    """A device allocation made by the library's own HIP runtime (so that no
+    second runtime is loaded into the process)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().cps_hip_device_alloc(ctypes.byref(p), self.nbytes), "cps_hip_device_alloc")
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        assert a.dtype == np.float64 and a.flags.c_contiguous and a.nbytes == self.nbytes
+        _check(lib().cps_hip_memcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), self.nbytes, 0), "cps_hip_memcpy")
+
+    def download(self):
+        a = np.empty(self.nbytes // 8)
+        _check(lib().cps_hip_memcpy(a.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes, 1), "cps_hip_memcpy")
+        return a
+
+    def copy_from(self, other):
+        assert other.nbytes == self.nbytes
+        _check(lib().cps_hip_memcpy(self.ptr, other.ptr, self.nbytes, 2), "cps_hip_memcpy")
+
+    def free(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib().cps_hip_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def device_count():

@@ -64,6 +64,35 @@ int cps_hip_device_count(void) {
 
 const char* cps_hip_version(void) { return "cps_hip 0.1 gfx950"; }
 
+int cps_hip_device_alloc(void** p, size_t bytes) {
+  if (!p || bytes == 0) return CPS_HIP_ERR_ARG;
+  *p = nullptr;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return CPS_HIP_ERR_NOMEM;
+  }
+  return 0;
+}
+
+int cps_hip_device_free(void* p) {
+  if (!p) return 0;
+  return hipFree(p) == hipSuccess ? 0 : CPS_HIP_ERR_RUNTIME;
+}
+
+int cps_hip_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  if ((!dst || !src) && bytes) return CPS_HIP_ERR_ARG;
+  hipMemcpyKind k;
+  switch (kind) {
+    case CPS_HIP_H2D: k = hipMemcpyHostToDevice; break;
+    case CPS_HIP_D2H: k = hipMemcpyDeviceToHost; break;
+    case CPS_HIP_D2D: k = hipMemcpyDeviceToDevice; break;
+    default: return CPS_HIP_ERR_ARG;
+  }
+  if (bytes == 0) return 0;
+  return hipMemcpy(dst, src, bytes, k) == hipSuccess ? 0 : CPS_HIP_ERR_RUNTIME;
+}
+
 int cps_hip_lattice_create(cps_hip_lattice** lat, const int node_sites[4], int ls, const int bc[4]) {
   if (!lat || !node_sites) return CPS_HIP_ERR_ARG;
   *lat = nullptr;
@@ -296,8 +325,8 @@ int cps_hip_clover_export(cps_hip_lattice* lat, int k, int cb, double* A) {
 }
 
 // ---- InvCg -----------------------------------------------------------------
-int cps_hip_invcg(cps_hip_lattice* lat, const cps_hip_op* op, double* sol, const double* src, int nrhs,
-                  const cps_hip_cg_arg* arg, cps_hip_cg_result* res) {
+static int invcg_common(cps_hip_lattice* lat, const cps_hip_op* op, double* sol, const double* src, int nrhs,
+                        const cps_hip_cg_arg* arg, cps_hip_cg_result* res, bool on_device) {
   if (check_lat(lat) || !op || !sol || !src || !arg || nrhs < 1) return CPS_HIP_ERR_ARG;
   Lattice& L = lat->L;
   if (!L.has_gauge) return CPS_HIP_ERR_NOGAUGE;
@@ -308,9 +337,9 @@ int cps_hip_invcg(cps_hip_lattice* lat, const cps_hip_op* op, double* sol, const
   Field& x = L.field(S_X, STORE_D, n);
   Field& b = L.field(S_SRC, STORE_D, n);
   if (L.last_rc) return L.last_rc;
-  rc = upload(L, src, b, o, 1, nrhs, 0);
+  rc = upload(L, src, b, o, 1, nrhs, 0, on_device);
   if (rc) return rc;
-  rc = upload(L, sol, x, o, 1, nrhs, 1);
+  rc = upload(L, sol, x, o, 1, nrhs, 1, on_device);
   if (rc) return rc;
   CPSHIP_CHECK(hipStreamSynchronize(L.stream));
   std::vector<int> iters, status, nup;
@@ -321,7 +350,7 @@ int cps_hip_invcg(cps_hip_lattice* lat, const cps_hip_op* op, double* sol, const
   if (rc) return rc;
   CPSHIP_CHECK(hipStreamSynchronize(L.stream));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  rc = download(L, x, sol, o, 1, nrhs);
+  rc = download(L, x, sol, o, 1, nrhs, on_device);
   if (rc) return rc;
   bool dzero = false;
   for (int k = 0; k < nrhs; k++) {
@@ -336,6 +365,16 @@ int cps_hip_invcg(cps_hip_lattice* lat, const cps_hip_op* op, double* sol, const
     if (status[k] == CG_STATUS_D_ZERO) dzero = true;
   }
   return dzero ? CPS_HIP_ERR_D_ZERO : 0;
+}
+
+int cps_hip_invcg(cps_hip_lattice* lat, const cps_hip_op* op, double* sol, const double* src, int nrhs,
+                  const cps_hip_cg_arg* arg, cps_hip_cg_result* res) {
+  return invcg_common(lat, op, sol, src, nrhs, arg, res, false);
+}
+
+int cps_hip_invcg_device(cps_hip_lattice* lat, const cps_hip_op* op, double* sol, const double* src, int nrhs,
+                         const cps_hip_cg_arg* arg, cps_hip_cg_result* res) {
+  return invcg_common(lat, op, sol, src, nrhs, arg, res, true);
 }
 
 // ---- MatInv ----------------------------------------------------------------

@@ -123,8 +123,11 @@ int clover_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& ou
                  const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs);
 int clover_build(Lattice& L, const double* kappas, int n_kappa, double csw);
 
-int upload(Lattice& L, const double* host, Field& f, const OpLayout& o, int cb5, int nrhs, int slot_io);
-int download(Lattice& L, const Field& f, double* host, const OpLayout& o, int cb5, int nrhs);
+// host <-> device field transfer with layout conversion; with on_device the
+// double pointer is device memory in the CPS layout (no PCIe copy)
+int upload(Lattice& L, const double* host, Field& f, const OpLayout& o, int cb5, int nrhs, int slot_io,
+           bool on_device = false);
+int download(Lattice& L, const Field& f, double* host, const OpLayout& o, int cb5, int nrhs, bool on_device = false);
 
 int cg_double_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int nrhs, const cps_hip_cg_arg* arg,
                      std::vector<int>& iters, std::vector<int>& status, std::vector<double>& true_res,

@@ -318,18 +318,28 @@ int apply_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out
 // ---------------------------------------------------------------------------
 // host <-> device vector transfer in CPS order
 
-int upload(Lattice& L, const double* host, Field& f, const OpLayout& o, int cb5, int nrhs, int slot_io) {
+int upload(Lattice& L, const double* host, Field& f, const OpLayout& o, int cb5, int nrhs, int slot_io,
+           bool on_device) {
   const int64_t n = o.per_rhs * nrhs;
-  DevBuf& io = slot_io ? L.io2 : L.io;
-  int rc = io.ensure((size_t)n * 24 * sizeof(double));
-  if (rc) return rc;
-  CPSHIP_CHECK(hipMemcpyAsync(io.p, host, (size_t)n * 24 * sizeof(double), hipMemcpyHostToDevice, L.stream));
-  CPSHIP_CHECK(pack_spinor(f.storage, (const double*)io.p, f.ref(), o.map, L.g.Vcb, L.ls, cb5, n, L.stream));
+  const double* src = host;
+  if (!on_device) {
+    DevBuf& io = slot_io ? L.io2 : L.io;
+    int rc = io.ensure((size_t)n * 24 * sizeof(double));
+    if (rc) return rc;
+    CPSHIP_CHECK(hipMemcpyAsync(io.p, host, (size_t)n * 24 * sizeof(double), hipMemcpyHostToDevice, L.stream));
+    src = (const double*)io.p;
+  }
+  CPSHIP_CHECK(pack_spinor(f.storage, src, f.ref(), o.map, L.g.Vcb, L.ls, cb5, n, L.stream));
   return 0;
 }
 
-int download(Lattice& L, const Field& f, double* host, const OpLayout& o, int cb5, int nrhs) {
+int download(Lattice& L, const Field& f, double* host, const OpLayout& o, int cb5, int nrhs, bool on_device) {
   const int64_t n = o.per_rhs * nrhs;
+  if (on_device) {
+    CPSHIP_CHECK(unpack_spinor(f.storage, f.ref(), host, o.map, L.g.Vcb, L.ls, cb5, n, L.stream));
+    CPSHIP_CHECK(hipStreamSynchronize(L.stream));
+    return 0;
+  }
   int rc = L.io.ensure((size_t)n * 24 * sizeof(double));
   if (rc) return rc;
   CPSHIP_CHECK(unpack_spinor(f.storage, f.ref(), (double*)L.io.p, o.map, L.g.Vcb, L.ls, cb5, n, L.stream));

@@ -27,6 +27,8 @@
 #ifndef CPS_HIP_H
 #define CPS_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -157,6 +159,11 @@ int cps_hip_clover_export(cps_hip_lattice *lat, int k, int cb, double *A);
  * sloppy CG with FP64 reliable updates and stop on the FP64 residual. */
 int cps_hip_invcg(cps_hip_lattice *lat, const cps_hip_op *op, double *sol, const double *src, int nrhs,
                   const cps_hip_cg_arg *arg, cps_hip_cg_result *res);
+/* The same solve with sol and src in device memory (CPS layout, FP64, on the
+ * current device): no PCIe transfer; sources and solutions stay in HBM.
+ * Same inputs, semantics and results as cps_hip_invcg. */
+int cps_hip_invcg_device(cps_hip_lattice *lat, const cps_hip_op *op, double *sol, const double *src, int nrhs,
+                         const cps_hip_cg_arg *arg, cps_hip_cg_result *res);
 
 /* ---- MatInv (d_op_wilson.C:257-334, d_op_dwf.C:316-509) ------------------
  * Full-vector solve M out = in by even-odd preconditioning + InvCg; in and
@@ -176,6 +183,16 @@ int cps_hip_bench_hop(cps_hip_lattice *lat, int kind, int precision, int nrhs, i
  * field) of n_elem spinors (192 bytes read + 192 bytes written each): the
  * known-size reference that scales hardware traffic counters. */
 int cps_hip_bench_copy(cps_hip_lattice *lat, long long n_elem, int reps, double *seconds);
+
+/* ---- device memory for device-resident callers ---------------------------
+ * Plain device buffers on the library's own HIP runtime, for callers that
+ * keep sources and solutions in HBM (cps_hip_invcg_device).  No counterpart
+ * in CPS, whose vectors are host memory (smalloc). */
+enum { CPS_HIP_H2D = 0, CPS_HIP_D2H = 1, CPS_HIP_D2D = 2 };
+int cps_hip_device_alloc(void **p, size_t bytes);
+int cps_hip_device_free(void *p);
+/* Synchronous copy; kind is CPS_HIP_H2D, CPS_HIP_D2H or CPS_HIP_D2D. */
+int cps_hip_memcpy(void *dst, const void *src, size_t bytes, int kind);
 
 int cps_hip_device_count(void);
 const char *cps_hip_version(void);

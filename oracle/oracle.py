@@ -92,6 +92,12 @@ def _declare(l):
         f = getattr(l, name)
         f.argtypes = args
         f.restype = ctypes.c_int if name.endswith(("invcg", "matinv")) else None
+    l.ora_dot.argtypes = [vp, vp, ctypes.c_size_t]
+    l.ora_dot.restype = ctypes.c_double
+    l.ora_ftimesv1plusv2.argtypes = [vp, ctypes.c_double, vp, vp, ctypes.c_size_t]
+    l.ora_ftimesv1plusv2.restype = None
+    l.ora_dwf_kappa.argtypes = [vp]
+    l.ora_dwf_kappa.restype = ctypes.c_double
 
 
 def _p(a):
@@ -165,6 +171,17 @@ def bond_cond(u_canon, L, bc):
     return u
 
 
+# ---- vector BLAS ---------------------------------------------------------
+def dot(a, b):
+    return lib().ora_dot(_p(a), _p(b), a.size)
+
+
+def ftimesv1plusv2(b, c, d):
+    out = np.empty_like(c)
+    lib().ora_ftimesv1plusv2(_p(out), b, _p(c), _p(d), c.size)
+    return out
+
+
 # ---- Wilson -------------------------------------------------------------
 def half(L):
     return 24 * vol(L) // 2
@@ -219,6 +236,10 @@ def dwf_struct(L, ls, m5, a5_inv=1.0):
     d.L[:] = [int(v) for v in L]
     d.ls, d.m5, d.a5_inv = ls, m5, a5_inv
     return d
+
+
+def dwf_kappa(d):
+    return lib().ora_dwf_kappa(ctypes.byref(d))
 
 
 def dwf_half(d):

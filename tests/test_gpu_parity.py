@@ -149,6 +149,25 @@ def test_dwf_multi_rhs_matches_single(dwf4):
         assert rel(batched[k * n:(k + 1) * n], single) < 1e-13
 
 
+def test_invcg_device_matches_host(dwf4):
+    """cps_hip_invcg_device (vectors resident in HBM) gives the host entry
+    point's result bit for bit, for FP64 and mixed precision, two rhs."""
+    lat, u, L, ls = dwf4
+    op = C.make_op(C.OP_DWF, mass=0.01, m5=1.8)
+    src = np.concatenate([F.gaussian_fermion(lat.vcb * ls, 90 + k) for k in range(2)])
+    for prec in (C.PREC_DOUBLE, C.PREC_MIXED_HALF):
+        arg = C.make_cg_arg(1e-10, 5000, precision=prec)
+        host, rh = lat.invcg(op, src, arg, nrhs=2)
+        s_dev = C.DeviceBuffer.from_array(src)
+        x_dev = C.DeviceBuffer.from_array(np.zeros_like(src))
+        rd = lat.invcg_device(op, x_dev, s_dev, arg, nrhs=2)
+        got = x_dev.download()
+        s_dev.free()
+        x_dev.free()
+        assert [r.iterations for r in rd] == [r.iterations for r in rh]
+        assert np.array_equal(got, host)
+
+
 @pytest.fixture(scope="module")
 def mob4():
     L = [4, 4, 4, 4]
