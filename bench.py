@@ -26,6 +26,8 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# the solver reports progress on stderr every 20 s during long solves
+os.environ.setdefault("CPS_HIP_PROGRESS", "20")
 sys.path.insert(0, ROOT)
 
 import cps_public_amd as C  # noqa: E402
@@ -137,7 +139,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     cfg = CONFIGS[a.config]
+    print(f"# {cfg['name']}: building inputs", file=sys.stderr, flush=True)
     u_w, src = make_inputs(cfg)
+    print("# inputs ready", file=sys.stderr, flush=True)
     lat = C.Lattice(cfg["L"], cfg["ls"], (0, 0, 0, 1))
     lat.import_gauge(u_w)
     op = op_for(cfg)

@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcps_hip.so")
+# CPS_HIP_LIB selects another build of the library (same ABI)
+LIB_PATH = os.environ.get("CPS_HIP_LIB") or os.path.join(_HERE, "libcps_hip.so")
 HEADER_PATH = os.path.join(_HERE, "..", "include", "cps_hip.h")
 
 OP_WILSON, OP_DWF, OP_MOBIUS, OP_CLOVER = 0, 1, 2, 3
@@ -37,7 +38,13 @@ class Op(ctypes.Structure):
 class CgArg(ctypes.Structure):
     _fields_ = [("max_num_iter", ctypes.c_int), ("stop_rsd", ctypes.c_double), ("precision", ctypes.c_int),
                 ("reliable_every", ctypes.c_int), ("reliable_delta", ctypes.c_double),
-                ("check_every", ctypes.c_int)]
+                ("check_every", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+
+class ServiceStats(ctypes.Structure):
+    _fields_ = [("requests_done", ctypes.c_longlong), ("rhs_done", ctypes.c_longlong),
+                ("iterations", ctypes.c_longlong), ("peak_bytes", ctypes.c_longlong),
+                ("engines_created", ctypes.c_int), ("engines_live", ctypes.c_int), ("last_error", ctypes.c_int)]
 
 
 class CgResult(ctypes.Structure):
@@ -86,6 +93,12 @@ def _declare(l):
         "cps_hip_bench_copy": [vp, ctypes.c_longlong, i, P(d)],
         "cps_hip_device_count": [],
         "cps_hip_device_alloc": [P(vp), ctypes.c_size_t],
+        "cps_hip_service_create": [P(vp), ctypes.c_longlong, i],
+        "cps_hip_service_destroy": [vp],
+        "cps_hip_service_add_gauge": [vp, vp, i, vp, vp, P(i)],
+        "cps_hip_service_submit": [vp, i, P(Op), vp, vp, i, d, i, P(ctypes.c_longlong)],
+        "cps_hip_service_wait": [vp, ctypes.c_longlong, P(CgResult), P(d)],
+        "cps_hip_service_get_stats": [vp, P(ServiceStats)],
         "cps_hip_device_free": [vp],
         "cps_hip_memcpy": [vp, vp, ctypes.c_size_t, i],
     }
@@ -113,8 +126,8 @@ def make_op(kind, mass=0.0, m5=1.8, a5_inv=1.0, b=1.0, c=0.0, csw=0.0):
 
 
 def make_cg_arg(stop_rsd=1e-10, max_num_iter=10000, precision=PREC_DOUBLE, reliable_every=100,
-                reliable_delta=0.1, check_every=0):
-    return CgArg(max_num_iter, stop_rsd, precision, reliable_every, reliable_delta, check_every)
+                reliable_delta=0.1, check_every=0, use_graph=1):
+    return CgArg(max_num_iter, stop_rsd, precision, reliable_every, reliable_delta, check_every, use_graph)
 
 
 class Lattice:
@@ -255,6 +268,65 @@ class Lattice:
         s = ctypes.c_double()
         _check(lib().cps_hip_bench_copy(self.h, n_elem, reps, ctypes.byref(s)), "cps_hip_bench_copy")
         return s.value
+
+
+class Service:
+    """Resident solve service (cps_hip_service_*): gauge fields stay on the
+    device; submit() queues a solve of nrhs sources and returns a request id
+    at once, wait() blocks for its solutions, per-rhs results and latency."""
+
+    def __init__(self, hbm_budget_bytes, max_slots=12):
+        h = ctypes.c_void_p()
+        _check(lib().cps_hip_service_create(ctypes.byref(h), int(hbm_budget_bytes), int(max_slots)),
+               "cps_hip_service_create")
+        self.h = h
+        self._gauge_ls = {}
+        self._live = {}
+
+    def add_gauge(self, node_sites, ls, u_wilson, bc=(0, 0, 0, 1)):
+        gid = ctypes.c_int()
+        dims = (ctypes.c_int * 4)(*[int(v) for v in node_sites])
+        bcs = (ctypes.c_int * 4)(*[int(v) for v in bc])
+        u = np.ascontiguousarray(u_wilson, dtype=np.float64)
+        _check(lib().cps_hip_service_add_gauge(self.h, dims, int(ls), bcs, _ptr(u), ctypes.byref(gid)),
+               "cps_hip_service_add_gauge")
+        self._gauge_ls[gid.value] = (list(node_sites), int(ls))
+        return gid.value
+
+    def submit(self, gauge_id, op, src, nrhs=1, stop_rsd=1e-10, max_num_iter=10000):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        sol = np.zeros_like(src)
+        rid = ctypes.c_longlong()
+        _check(lib().cps_hip_service_submit(self.h, int(gauge_id), ctypes.byref(op), _ptr(src), _ptr(sol), int(nrhs),
+                                            float(stop_rsd), int(max_num_iter), ctypes.byref(rid)),
+               "cps_hip_service_submit")
+        self._live[rid.value] = (src, sol, nrhs)   # buffers stay alive until wait()
+        return rid.value
+
+    def wait(self, request_id):
+        src, sol, nrhs = self._live[request_id]
+        res = (CgResult * nrhs)()
+        lat = ctypes.c_double()
+        rc = lib().cps_hip_service_wait(self.h, ctypes.c_longlong(request_id), res, ctypes.byref(lat))
+        del self._live[request_id]
+        _check(rc, "cps_hip_service_wait")
+        return sol, [res[k] for k in range(nrhs)], lat.value
+
+    def stats(self):
+        st = ServiceStats()
+        _check(lib().cps_hip_service_get_stats(self.h, ctypes.byref(st)), "cps_hip_service_get_stats")
+        return {f: getattr(st, f) for f, _ in ServiceStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cps_hip_service_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:

@@ -8,12 +8,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcps_hip.so")
-SOURCES = ["hop.hip", "layout.hip", "blas.hip", "solver.hip", "mobius.hip", "clover.hip", "capi.cpp"]
+SOURCES = ["hop.hip", "hop_clover.hip", "layout.hip", "blas.hip", "solver.hip", "mobius.hip", "clover.hip", "capi.cpp", "service.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CPS_HIP_ARCH", "gfx950")
 FLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
-    # packed FP32 instructions stay off (build notes)
+    # no packed FP32 (v_pk_*_f32) instructions are generated
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include"),
 ]

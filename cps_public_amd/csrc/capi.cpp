@@ -316,12 +316,8 @@ int cps_hip_clover_build(cps_hip_lattice* lat, const double* kappas, int n_kappa
 int cps_hip_clover_export(cps_hip_lattice* lat, int k, int cb, double* A) {
   if (check_lat(lat) || !A || k < 0 || k >= lat->L.n_clover || (cb != 0 && cb != 1)) return CPS_HIP_ERR_ARG;
   Lattice& L = lat->L;
-  const size_t per = (size_t)72 * L.g.Vcb;
-  // device layout [k][cb][site][72]
-  const double* src = (const double*)L.clover.p + ((size_t)k * 2 + cb) * per;
-  CPSHIP_CHECK(hipMemcpyAsync(A, src, per * sizeof(double), hipMemcpyDeviceToHost, L.stream));
   CPSHIP_CHECK(hipStreamSynchronize(L.stream));
-  return 0;
+  return clover_export(L, k, cb, A);
 }
 
 // ---- InvCg -----------------------------------------------------------------

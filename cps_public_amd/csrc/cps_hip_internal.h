@@ -54,6 +54,12 @@ __device__ __forceinline__ cplx<R> cmk(R a, R b) { cplx<R> c; c.re = a; c.im = b
 
 // a*b
 template <typename R>
+__device__ __forceinline__ cplx<R> cadd(cplx<R> a, cplx<R> b) { return cmk<R>(a.re + b.re, a.im + b.im); }
+template <typename R>
+__device__ __forceinline__ cplx<R> csub(cplx<R> a, cplx<R> b) { return cmk<R>(a.re - b.re, a.im - b.im); }
+template <typename R>
+__device__ __forceinline__ cplx<R> cconj(cplx<R> a) { return cmk<R>(a.re, -a.im); }
+template <typename R>
 __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
   return cmk<R>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
@@ -406,10 +412,133 @@ __device__ __forceinline__ void link_decode(const LinkRaw<LK>& r, R sign, cplx<R
   }
 }
 
-// how many hop directions the kernel loads before decoding (register budget)
+// how many hop directions the kernel loads before decoding them (1, 2, 4 or
+// 8): more loads in flight per thread against more registers per thread
+#ifndef CPSHIP_HOP_BATCH_D
+#define CPSHIP_HOP_BATCH_D 2
+#endif
+#ifndef CPSHIP_HOP_BATCH_F
+#define CPSHIP_HOP_BATCH_F 4
+#endif
+#ifndef CPSHIP_HOP_BATCH_H
+#define CPSHIP_HOP_BATCH_H 4
+#endif
 template <typename S> struct HopBatch;
-template <> struct HopBatch<SpinD> { static constexpr int value = 2; };
-template <> struct HopBatch<SpinF> { static constexpr int value = 4; };
-template <> struct HopBatch<SpinH> { static constexpr int value = 8; };
+template <> struct HopBatch<SpinD> { static constexpr int value = CPSHIP_HOP_BATCH_D; };
+template <> struct HopBatch<SpinF> { static constexpr int value = CPSHIP_HOP_BATCH_F; };
+template <> struct HopBatch<SpinH> { static constexpr int value = CPSHIP_HOP_BATCH_H; };
+
+// ---- clover field -----------------------------------------------------------
+// 72 reals per site: two 6x6 Hermitian chiral blocks (block 0 acts on spins
+// 0,1, block 1 on spins 2,3; components spin*3+colour), each packed in 36
+// reals as CPS's clover_mat_mlt reads them (clover.C:127-172): the real
+// diagonal (i,i) at i*i+2i, the lower element (i,j), i>j, at i*i+2j as
+// (re,im).  On the device a field of one kappa and one checkerboard is a set
+// of planes of 16-byte chunks, one chunk per site per plane: FP64 36 planes
+// of double2, FP32 18 planes of float4, FP16 9 planes of 8 halves.  Sets are
+// ordered [kappa][cb] (cb 0: A_ee^-1, cb 1: A_oo).
+enum CloverStorage { CLOVER_D = 0, CLOVER_F = 1, CLOVER_H = 2, NCLOVER = 3 };
+template <typename S> struct CloverOf;
+template <> struct CloverOf<SpinD> { static constexpr int storage = CLOVER_D, nplanes = 36; };
+template <> struct CloverOf<SpinF> { static constexpr int storage = CLOVER_F, nplanes = 18; };
+template <> struct CloverOf<SpinH> { static constexpr int storage = CLOVER_H, nplanes = 9; };
+inline int clover_nplanes(int storage) { return storage == CLOVER_D ? 36 : storage == CLOVER_F ? 18 : 9; }
+
+struct CloverRef {
+  const void* data;
+  int64_t plane;    // chunks per plane (= Vcb)
+};
+
+// The 36 reals of chiral block B of the site whose chunk index (set base +
+// site) is c0.
+template <typename S, typename R, int B>
+__device__ __forceinline__ void clover_load_block(const CloverRef& c, int64_t c0, R (&a)[36]) {
+  if constexpr (std::is_same<S, SpinD>::value) {
+    const double2* p = reinterpret_cast<const double2*>(c.data) + c0 + (int64_t)(18 * B) * c.plane;
+#pragma unroll
+    for (int q = 0; q < 18; q++) {
+      const double2 h = p[q * c.plane];
+      a[2 * q] = R(h.x);
+      a[2 * q + 1] = R(h.y);
+    }
+  } else if constexpr (std::is_same<S, SpinF>::value) {
+    const float4* p = reinterpret_cast<const float4*>(c.data) + c0 + (int64_t)(9 * B) * c.plane;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      const float4 h = p[q * c.plane];
+      a[4 * q] = R(h.x);
+      a[4 * q + 1] = R(h.y);
+      a[4 * q + 2] = R(h.z);
+      a[4 * q + 3] = R(h.w);
+    }
+  } else {
+    // 72 halves in 9 chunks: block 0 is chunks 0-3 and the first half of
+    // chunk 4, block 1 the second half of chunk 4 and chunks 5-8
+    const uint4* p = reinterpret_cast<const uint4*>(c.data) + c0;
+    uint32_t w[18];
+    const uint4 mid = p[4 * c.plane];
+    if constexpr (B == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint4 h = p[q * c.plane];
+        w[4 * q] = h.x; w[4 * q + 1] = h.y; w[4 * q + 2] = h.z; w[4 * q + 3] = h.w;
+      }
+      w[16] = mid.x;
+      w[17] = mid.y;
+    } else {
+      w[0] = mid.z;
+      w[1] = mid.w;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint4 h = p[(5 + q) * c.plane];
+        w[2 + 4 * q] = h.x; w[3 + 4 * q] = h.y; w[4 + 4 * q] = h.z; w[5 + 4 * q] = h.w;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 18; k++) {
+      const float2 f = h2f(w[k]);
+      a[2 * k] = R(f.x);
+      a[2 * k + 1] = R(f.y);
+    }
+  }
+}
+
+// y = A x for one packed 6x6 Hermitian block (components O..O+5 of x, y).
+template <typename R, int O>
+__device__ __forceinline__ void clover_block_mul(const R (&a)[36], const cplx<R> (&x)[12], cplx<R> (&y)[12]) {
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    const R d = a[i * i + 2 * i];
+    y[O + i] = cmk<R>(d * x[O + i].re, d * x[O + i].im);
+  }
+#pragma unroll
+  for (int i = 1; i < 6; i++)
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      if (j < i) {
+        const R lr = a[i * i + 2 * j], li = a[i * i + 2 * j + 1];
+        y[O + i].re += lr * x[O + j].re - li * x[O + j].im;
+        y[O + i].im += lr * x[O + j].im + li * x[O + j].re;
+        y[O + j].re += lr * x[O + i].re + li * x[O + i].im;
+        y[O + j].im += lr * x[O + i].im - li * x[O + i].re;
+      }
+    }
+}
+
+// y = A x for a full spinor (both chiral blocks) at chunk index c0.
+template <typename S, typename R>
+__device__ __forceinline__ void clover_mul(const CloverRef& c, int64_t c0, const cplx<R> (&x)[12], cplx<R> (&y)[12]) {
+  {
+    R a[36];
+    clover_load_block<S, R, 0>(c, c0, a);
+    clover_block_mul<R, 0>(a, x, y);
+  }
+  {
+    R a[36];
+    clover_load_block<S, R, 1>(c, c0, a);
+    clover_block_mul<R, 6>(a, x, y);
+  }
+}
+
 
 }  // namespace cpship

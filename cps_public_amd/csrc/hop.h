@@ -42,9 +42,23 @@ struct HopParams {
   FieldRef in, out, X;
   GaugeRef gauge;
   double* partial;   // per-block |out|^2, or nullptr
+  // clover epilogue (Wilson kind only): 0 none, CLOV_POST out = A (scale*hop),
+  // CLOV_X out = xcoef*(A X) + kscale[set]*scale*hop, with A the clover set of
+  // kappa index set = rhs / rhs_per_set at the output checkerboard and kscale
+  // a device array (required for CLOV_X)
+  const uint4* nbr;         // neighbour table (k_build_nbr)
+  const uint32_t* nbr_neg;
+  int clov;
+  int rhs_per_set;
+  CloverRef clover;
+  const double* kscale;
 };
 
+enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2 };
+
 hipError_t launch_hop(const HopParams& P, hipStream_t st);
+hipError_t build_nbr(const Geom& g, uint4* nbr, uint32_t* nbr_neg, hipStream_t st);
+hipError_t launch_hop_clover(const HopParams& P, hipStream_t st);
 
 // link storage read by a hop precision
 inline int link_storage_of(int hprec) {

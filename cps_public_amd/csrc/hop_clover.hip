@@ -1,0 +1,44 @@
+// Launchers of the Wilson hopping kernel with a fused clover epilogue
+// (out = A (scale hop) or out = xcoef A X + scale hop), the two halves of the
+// clover preconditioned operator A_oo - kappa^2 D A_ee^-1 D
+// (DiracOpClover::MatPcDagOrNot, d_op_clover.C:194-248).
+#include "hop_kernel.h"
+
+namespace cpship {
+
+template <typename SIN, typename SOUT, typename LK, typename R>
+static hipError_t launch_c(const HopParams& P, hipStream_t st) {
+  if (P.kind != KIND_WILSON || P.rhs_per_set < 1 || !P.clover.data) return hipErrorInvalidValue;
+  if (P.clov == CLOV_X && !P.kscale) return hipErrorInvalidValue;
+  const int64_t nelem = (int64_t)P.nrhs * P.nsites_out * P.nb;
+  const int bs = P.block;
+  const int64_t nblk = (nelem + bs - 1) / bs;
+  dim3 grid((unsigned)nblk), block(bs);
+  const bool nt = (P.flags & HOP_NT_STORE) != 0;
+#define CPSHIP_HOP_LAUNCH(DAG, CLOV)                                                                         \
+  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, true, CLOV>), grid, block, 0, st, P); \
+  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, false, CLOV>), grid, block, 0, st, P);
+  if (P.clov == CLOV_POST) {
+    if (P.dag) { CPSHIP_HOP_LAUNCH(1, CLOV_POST) } else { CPSHIP_HOP_LAUNCH(0, CLOV_POST) }
+  } else if (P.clov == CLOV_X) {
+    if (P.dag) { CPSHIP_HOP_LAUNCH(1, CLOV_X) } else { CPSHIP_HOP_LAUNCH(0, CLOV_X) }
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef CPSHIP_HOP_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_hop_clover(const HopParams& P, hipStream_t st) {
+  switch (P.prec) {
+    case PREC_D_D18: return launch_c<SpinD, SpinD, LinkD18, double>(P, st);
+    case PREC_F_F12: return launch_c<SpinF, SpinF, LinkF12, float>(P, st);
+    case PREC_H_H12: return launch_c<SpinH, SpinH, LinkH12, float>(P, st);
+    case PREC_H_B12: return launch_c<SpinH, SpinH, LinkB12, float>(P, st);
+    case PREC_H_H18: return launch_c<SpinH, SpinH, LinkH18, float>(P, st);
+    case PREC_H_B18: return launch_c<SpinH, SpinH, LinkB18, float>(P, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace cpship

@@ -1,0 +1,659 @@
+// Wilson / domain-wall hopping kernel template for gfx950 (instantiated in
+// hop.hip, and with the clover epilogues in hop_clover.hip).
+//
+// One thread per output element (rhs, site, j).  The 4-D hop gathers the 8
+// neighbour spinors, spin-projects each to a half spinor (2 spins x 3
+// colours), multiplies by U or U^dagger and reconstructs the lower spins;
+// links are shared by every j (and every rhs) of a site, so lanes of one
+// wavefront that belong to one site read them with one broadcast load.  The
+// domain-wall kernel adds the fifth-dimension hop from the same site's
+// opposite-chirality slots.  An epilogue forms out = xcoef*X + scale*hop and
+// optionally the per-rhs |out|^2 partial sums (wavefront shuffles, then one
+// LDS pass per block, one value per block written; a second kernel sums the
+// block values in a fixed order so results are reproducible).
+//
+// Operator conventions follow the CPS noarch code:
+//   chi(x) = sum_mu U_mu(x) (1 - s gamma_mu) psi(x+mu)
+//          + U_mu(x-mu)^dagger (1 + s gamma_mu) psi(x-mu),  s = +1 (dag 0), -1 (dag 1)
+//   (wilson_dslash.C:173-515; gamma matrices of OneMinusPlusGammaTimesPsi,
+//   wilson_dslash.C:64-128), and for domain wall
+//   out(s) += (2/a5) P+ in(s-1) + (2/a5) P- in(s+1), -mf at the walls,
+//   P+ and P- exchanged for dag 1 (dwf_dslash_5_plus.C:111-340).
+#pragma once
+#include "cps_hip_internal.h"
+#include "hop.h"
+
+namespace cpship {
+
+// (1 + A gamma_mu) psi projected to spins (0,1): h[0..2] spin 0, h[3..5] spin 1
+template <int MU, int A, typename R>
+__device__ __forceinline__ void project(const cplx<R> p[12], cplx<R> h[6]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const cplx<R> p0 = p[c], p1 = p[3 + c], p2 = p[6 + c], p3 = p[9 + c];
+    if constexpr (MU == 0) {        // h0 = p0 + A i p3, h1 = p1 + A i p2
+      h[c] = cmk<R>(p0.re - A * p3.im, p0.im + A * p3.re);
+      h[3 + c] = cmk<R>(p1.re - A * p2.im, p1.im + A * p2.re);
+    } else if constexpr (MU == 1) { // h0 = p0 - A p3, h1 = p1 + A p2
+      h[c] = cmk<R>(p0.re - A * p3.re, p0.im - A * p3.im);
+      h[3 + c] = cmk<R>(p1.re + A * p2.re, p1.im + A * p2.im);
+    } else if constexpr (MU == 2) { // h0 = p0 + A i p2, h1 = p1 - A i p3
+      h[c] = cmk<R>(p0.re - A * p2.im, p0.im + A * p2.re);
+      h[3 + c] = cmk<R>(p1.re + A * p3.im, p1.im - A * p3.re);
+    } else {                        // h0 = p0 + A p2, h1 = p1 + A p3
+      h[c] = cmk<R>(p0.re + A * p2.re, p0.im + A * p2.im);
+      h[3 + c] = cmk<R>(p1.re + A * p3.re, p1.im + A * p3.im);
+    }
+  }
+}
+
+// acc += full spinor rebuilt from the colour-rotated half spinor w
+template <int MU, int A, typename R>
+__device__ __forceinline__ void reconstruct(cplx<R> acc[12], const cplx<R> w[6]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const cplx<R> w0 = w[c], w1 = w[3 + c];
+    acc[c].re += w0.re; acc[c].im += w0.im;
+    acc[3 + c].re += w1.re; acc[3 + c].im += w1.im;
+    if constexpr (MU == 0) {        // r2 = -A i w1, r3 = -A i w0
+      acc[6 + c].re += A * w1.im; acc[6 + c].im -= A * w1.re;
+      acc[9 + c].re += A * w0.im; acc[9 + c].im -= A * w0.re;
+    } else if constexpr (MU == 1) { // r2 = A w1, r3 = -A w0
+      acc[6 + c].re += A * w1.re; acc[6 + c].im += A * w1.im;
+      acc[9 + c].re -= A * w0.re; acc[9 + c].im -= A * w0.im;
+    } else if constexpr (MU == 2) { // r2 = -A i w0, r3 = A i w1
+      acc[6 + c].re += A * w0.im; acc[6 + c].im -= A * w0.re;
+      acc[9 + c].re -= A * w1.im; acc[9 + c].im += A * w1.re;
+    } else {                        // r2 = A w0, r3 = A w1
+      acc[6 + c].re += A * w0.re; acc[6 + c].im += A * w0.im;
+      acc[9 + c].re += A * w1.re; acc[9 + c].im += A * w1.im;
+    }
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void su3_mul(const cplx<R> u[9], const cplx<R> h[6], cplx<R> w[6]) {
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      cplx<R> a = cmul(u[3 * r + 0], h[3 * s + 0]);
+      cplx<R> b = cmul(u[3 * r + 1], h[3 * s + 1]);
+      cplx<R> c = cmul(u[3 * r + 2], h[3 * s + 2]);
+      w[3 * s + r] = cmk<R>(a.re + b.re + c.re, a.im + b.im + c.im);
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void su3_dag_mul(const cplx<R> u[9], const cplx<R> h[6], cplx<R> w[6]) {
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      cplx<R> a = cmulc(u[0 * 3 + r], h[3 * s + 0]);
+      cplx<R> b = cmulc(u[1 * 3 + r], h[3 * s + 1]);
+      cplx<R> c = cmulc(u[2 * 3 + r], h[3 * s + 2]);
+      w[3 * s + r] = cmk<R>(a.re + b.re + c.re, a.im + b.im + c.im);
+    }
+}
+
+struct Coord {
+  int x[4];
+};
+
+__device__ __forceinline__ Coord site_coords(const Geom& g, int site_cb, int parity) {
+  Coord c;
+  const int lh = g.L[0] >> 1;
+  int r = site_cb / lh;
+  const int xh = site_cb - r * lh;
+  c.x[1] = r % g.L[1];
+  r /= g.L[1];
+  c.x[2] = r % g.L[2];
+  c.x[3] = r / g.L[2];
+  c.x[0] = 2 * xh + ((c.x[1] + c.x[2] + c.x[3] + parity) & 1);
+  return c;
+}
+
+__device__ __forceinline__ int cb_index(const Geom& g, const int x[4]) {
+  return (x[0] >> 1) + (g.L[0] >> 1) * (x[1] + g.L[1] * (x[2] + g.L[2] * x[3]));
+}
+
+// Neighbours of one output site: direction d = 0..3 is +mu (mu = d), d = 4..7
+// is -mu (mu = d-4), the CPS summation order (wilson_dslash.C:437-447).
+struct Nbr {
+  int idx[8];      // checkerboard index of the neighbour site
+  unsigned neg;    // bit d: the link of direction d carries the boundary sign
+};
+
+__device__ __forceinline__ Nbr neighbours(const Geom& g, const Coord& c) {
+  Nbr N;
+  N.neg = 0;
+#pragma unroll
+  for (int d = 0; d < 8; d++) {
+    const int mu = d & 3;
+    int n[4] = {c.x[0], c.x[1], c.x[2], c.x[3]};
+    if (d < 4) {
+      n[mu] = (n[mu] + 1 == g.L[mu]) ? 0 : n[mu] + 1;
+      if (g.bc[mu] && c.x[mu] == g.L[mu] - 1) N.neg |= 1u << d;
+    } else {
+      n[mu] = (n[mu] == 0) ? g.L[mu] - 1 : n[mu] - 1;
+      if (g.bc[mu] && c.x[mu] == 0) N.neg |= 1u << d;
+    }
+    N.idx[d] = cb_index(g, n);
+  }
+  return N;
+}
+
+// ---- packed half-precision path ----------------------------------------------
+// FP16 spinors are projected in packed f16 arithmetic on their normalised
+// values, and the SU(3) product uses v_dot2 (two f16 or bf16 products added
+// into an f32 accumulator per instruction); the site norm is applied once to
+// the colour-rotated half spinor.  This keeps the sloppy hop within the
+// vector-ALU budget of the HBM-bound roofline.
+typedef _Float16 hv2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bv2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ hv2 as_hv2(uint32_t w) { return __builtin_bit_cast(hv2, w); }
+__device__ __forceinline__ bv2 as_bv2(uint32_t w) { return __builtin_bit_cast(bv2, w); }
+__device__ __forceinline__ hv2 hswap(hv2 a) { return __builtin_shufflevector(a, a, 1, 0); }
+
+template <int MU, int A>
+__device__ __forceinline__ void project_h(const SpinRaw<SpinH>& r, hv2 h[6]) {
+  const uint32_t p[12] = {r.q[0].x, r.q[0].y, r.q[0].z, r.q[0].w, r.q[1].x, r.q[1].y,
+                          r.q[1].z, r.q[1].w, r.q[2].x, r.q[2].y, r.q[2].z, r.q[2].w};
+  const hv2 ia = {(_Float16)(-A), (_Float16)(A)};   // A*i*q = swap(q) * (-A, A)
+  const hv2 aa = {(_Float16)(A), (_Float16)(A)};
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const hv2 p0 = as_hv2(p[c]), p1 = as_hv2(p[3 + c]), p2 = as_hv2(p[6 + c]), p3 = as_hv2(p[9 + c]);
+    if constexpr (MU == 0) {        // h0 = p0 + A i p3, h1 = p1 + A i p2
+      h[c] = p0 + hswap(p3) * ia;
+      h[3 + c] = p1 + hswap(p2) * ia;
+    } else if constexpr (MU == 1) { // h0 = p0 - A p3, h1 = p1 + A p2
+      h[c] = p0 - p3 * aa;
+      h[3 + c] = p1 + p2 * aa;
+    } else if constexpr (MU == 2) { // h0 = p0 + A i p2, h1 = p1 - A i p3
+      h[c] = p0 + hswap(p2) * ia;
+      h[3 + c] = p1 - hswap(p3) * ia;
+    } else {                        // h0 = p0 + A p2, h1 = p1 + A p3
+      h[c] = p0 + p2 * aa;
+      h[3 + c] = p1 + p3 * aa;
+    }
+  }
+}
+
+// acc += n * (full spinor rebuilt from w)
+template <int MU, int A>
+__device__ __forceinline__ void reconstruct_scaled(cplx<float> acc[12], const cplx<float> w[6], float n) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const cplx<float> w0 = w[c], w1 = w[3 + c];
+    acc[c].re += n * w0.re; acc[c].im += n * w0.im;
+    acc[3 + c].re += n * w1.re; acc[3 + c].im += n * w1.im;
+    const float an = A * n;
+    if constexpr (MU == 0) {
+      acc[6 + c].re += an * w1.im; acc[6 + c].im -= an * w1.re;
+      acc[9 + c].re += an * w0.im; acc[9 + c].im -= an * w0.re;
+    } else if constexpr (MU == 1) {
+      acc[6 + c].re += an * w1.re; acc[6 + c].im += an * w1.im;
+      acc[9 + c].re -= an * w0.re; acc[9 + c].im -= an * w0.im;
+    } else if constexpr (MU == 2) {
+      acc[6 + c].re += an * w0.im; acc[6 + c].im -= an * w0.re;
+      acc[9 + c].re -= an * w1.im; acc[9 + c].im += an * w1.re;
+    } else {
+      acc[6 + c].re += an * w0.re; acc[6 + c].im += an * w0.im;
+      acc[9 + c].re += an * w1.re; acc[9 + c].im += an * w1.im;
+    }
+  }
+}
+
+// third row of a 12-real link, sign * conj(row0 x row1), from f32 values
+__device__ __forceinline__ void row2_f32(const float2 u[6], float sign, float2 r[3]) {
+  r[0] = make_float2(sign * (u[1].x * u[5].x - u[1].y * u[5].y - u[2].x * u[4].x + u[2].y * u[4].y),
+                     -sign * (u[1].x * u[5].y + u[1].y * u[5].x - u[2].x * u[4].y - u[2].y * u[4].x));
+  r[1] = make_float2(sign * (u[2].x * u[3].x - u[2].y * u[3].y - u[0].x * u[5].x + u[0].y * u[5].y),
+                     -sign * (u[2].x * u[3].y + u[2].y * u[3].x - u[0].x * u[5].y - u[0].y * u[5].x));
+  r[2] = make_float2(sign * (u[0].x * u[4].x - u[0].y * u[4].y - u[1].x * u[3].x + u[1].y * u[3].y),
+                     -sign * (u[0].x * u[4].y + u[0].y * u[4].x - u[1].x * u[3].y - u[1].y * u[3].x));
+}
+
+// link elements as packed 32-bit words (f16 or bf16 pairs), third row rebuilt for 12-real storage
+template <typename LK>
+__device__ __forceinline__ void link_words(const LinkRaw<LK>& r, float sign, uint32_t U[9]) {
+  if constexpr (LinkIs18Half<LK>::value) {
+    U[0] = r.a.x; U[1] = r.a.y; U[2] = r.a.z; U[3] = r.a.w;
+    U[4] = r.b.x; U[5] = r.b.y; U[6] = r.b.z; U[7] = r.b.w; U[8] = r.c;
+    (void)sign;
+  } else {
+    U[0] = r.a.x; U[1] = r.a.y; U[2] = r.a.z; U[3] = r.a.w; U[4] = r.b.x; U[5] = r.b.y;
+    float2 f[6], r2[3];
+#pragma unroll
+    for (int k = 0; k < 6; k++) f[k] = LinkIsBf16<LK>::value ? bf2f(U[k]) : h2f(U[k]);
+    row2_f32(f, sign, r2);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if constexpr (LinkIsBf16<LK>::value) {
+        const bv2 b = {(__bf16)r2[k].x, (__bf16)r2[k].y};
+        U[6 + k] = __builtin_bit_cast(uint32_t, b);
+      } else {
+        U[6 + k] = f2h(r2[k].x, r2[k].y);
+      }
+    }
+  }
+}
+
+// One v_pk_add_f16 per needed operand: h = p + alpha q (alpha = +1, -1, +i, -i
+// for ALPHA = 0..3) delivered directly in the form the dot2 product wants:
+//   VAR 0: (h.re, -h.im)   VAR 1: (h.im, h.re)      (U h)
+//   VAR 2: (h.re,  h.im)   VAR 3: (h.im, -h.re)     (U^dagger h)
+// The half selection and negations are instruction modifiers, so the
+// projection costs 12 packed instructions per direction.
+#define CPSHIP_PKADD(OS0, OS1, OH0, OH1, NL0, NL1, NH0, NH1)                                                  \
+  asm("v_pk_add_f16 %0, %1, %2 op_sel:[" #OS0 "," #OS1 "] op_sel_hi:[" #OH0 "," #OH1 "] neg_lo:[" #NL0 "," #NL1 \
+      "] neg_hi:[" #NH0 "," #NH1 "]"                                                                            \
+      : "=v"(r)                                                                                                \
+      : "v"(p), "v"(q))
+
+template <int ALPHA, int VAR>
+__device__ __forceinline__ uint32_t proj_word(uint32_t p, uint32_t q) {
+  uint32_t r;
+  if constexpr (ALPHA == 0) {          // h = (p.x + q.x, p.y + q.y)
+    if constexpr (VAR == 0) CPSHIP_PKADD(0, 0, 1, 1, 0, 0, 1, 1);
+    else if constexpr (VAR == 1) CPSHIP_PKADD(1, 1, 0, 0, 0, 0, 0, 0);
+    else if constexpr (VAR == 2) CPSHIP_PKADD(0, 0, 1, 1, 0, 0, 0, 0);
+    else CPSHIP_PKADD(1, 1, 0, 0, 0, 0, 1, 1);
+  } else if constexpr (ALPHA == 1) {   // h = (p.x - q.x, p.y - q.y)
+    if constexpr (VAR == 0) CPSHIP_PKADD(0, 0, 1, 1, 0, 1, 1, 0);
+    else if constexpr (VAR == 1) CPSHIP_PKADD(1, 1, 0, 0, 0, 1, 0, 1);
+    else if constexpr (VAR == 2) CPSHIP_PKADD(0, 0, 1, 1, 0, 1, 0, 1);
+    else CPSHIP_PKADD(1, 1, 0, 0, 0, 1, 1, 0);
+  } else if constexpr (ALPHA == 2) {   // h = (p.x - q.y, p.y + q.x)
+    if constexpr (VAR == 0) CPSHIP_PKADD(0, 1, 1, 0, 0, 1, 1, 1);
+    else if constexpr (VAR == 1) CPSHIP_PKADD(1, 0, 0, 1, 0, 0, 0, 1);
+    else if constexpr (VAR == 2) CPSHIP_PKADD(0, 1, 1, 0, 0, 1, 0, 0);
+    else CPSHIP_PKADD(1, 0, 0, 1, 0, 0, 1, 0);
+  } else {                             // h = (p.x + q.y, p.y - q.x)
+    if constexpr (VAR == 0) CPSHIP_PKADD(0, 1, 1, 0, 0, 0, 1, 0);
+    else if constexpr (VAR == 1) CPSHIP_PKADD(1, 0, 0, 1, 0, 1, 0, 0);
+    else if constexpr (VAR == 2) CPSHIP_PKADD(0, 1, 1, 0, 0, 0, 0, 1);
+    else CPSHIP_PKADD(1, 0, 0, 1, 0, 1, 1, 1);
+  }
+  return r;
+}
+#undef CPSHIP_PKADD
+
+// alpha of h0 = p0 + alpha0 q0 and h1 = p1 + alpha1 q1 for (1 + A gamma_mu),
+// with q0, q1 the spin-3/2 or spin-2/3 partners (see project_h)
+template <int MU, int A> struct ProjAlpha;
+template <int A> struct ProjAlpha<0, A> { static constexpr int a0 = A > 0 ? 2 : 3, a1 = A > 0 ? 2 : 3, q0 = 3, q1 = 2; };
+template <int A> struct ProjAlpha<1, A> { static constexpr int a0 = A > 0 ? 1 : 0, a1 = A > 0 ? 0 : 1, q0 = 3, q1 = 2; };
+template <int A> struct ProjAlpha<2, A> { static constexpr int a0 = A > 0 ? 2 : 3, a1 = A > 0 ? 3 : 2, q0 = 2, q1 = 3; };
+template <int A> struct ProjAlpha<3, A> { static constexpr int a0 = A > 0 ? 0 : 1, a1 = A > 0 ? 0 : 1, q0 = 2, q1 = 3; };
+
+__device__ __forceinline__ float dot2_first(uint32_t a, uint32_t b) {
+  float r;
+  asm("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// link words as f16 pairs (bf16 storage is widened to f16 here)
+template <typename LK>
+__device__ __forceinline__ void link_words_f16(const LinkRaw<LK>& r, float sign, uint32_t U[9]) {
+  link_words<LK>(r, sign, U);
+  if constexpr (LinkIsBf16<LK>::value) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const float2 f = bf2f(U[k]);
+      uint32_t h;
+      asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(h) : "v"(f.x), "v"(f.y));
+      U[k] = h;
+    }
+  }
+}
+
+template <int D, typename LK, int DAG>
+__device__ __forceinline__ void dir_compute_h2(const SpinRaw<SpinH>& sr, const LinkRaw<LK>& lr, float sign,
+                                               cplx<float> acc[12]) {
+  constexpr int MU = D & 3;
+  constexpr int S = (DAG == 0) ? 1 : -1;
+  constexpr int A = (D < 4) ? -S : S;
+  using PA = ProjAlpha<MU, A>;
+  constexpr int V1 = (D < 4) ? 0 : 2, V2 = (D < 4) ? 1 : 3;
+  const uint32_t p[12] = {sr.q[0].x, sr.q[0].y, sr.q[0].z, sr.q[0].w, sr.q[1].x, sr.q[1].y,
+                          sr.q[1].z, sr.q[1].w, sr.q[2].x, sr.q[2].y, sr.q[2].z, sr.q[2].w};
+  uint32_t hb[6], hx[6];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    hb[c] = proj_word<PA::a0, V1>(p[c], p[3 * PA::q0 + c]);
+    hx[c] = proj_word<PA::a0, V2>(p[c], p[3 * PA::q0 + c]);
+    hb[3 + c] = proj_word<PA::a1, V1>(p[3 + c], p[3 * PA::q1 + c]);
+    hx[3 + c] = proj_word<PA::a1, V2>(p[3 + c], p[3 * PA::q1 + c]);
+  }
+  uint32_t U[9];
+  link_words_f16<LK>(lr, sign, U);
+  cplx<float> w[6];
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      const uint32_t u0 = D < 4 ? U[3 * r + 0] : U[0 * 3 + r];
+      const uint32_t u1 = D < 4 ? U[3 * r + 1] : U[1 * 3 + r];
+      const uint32_t u2 = D < 4 ? U[3 * r + 2] : U[2 * 3 + r];
+      float re = dot2_first(u0, hb[3 * s + 0]);
+      float im = dot2_first(u0, hx[3 * s + 0]);
+      re = __builtin_amdgcn_fdot2(as_hv2(u1), as_hv2(hb[3 * s + 1]), re, false);
+      im = __builtin_amdgcn_fdot2(as_hv2(u1), as_hv2(hx[3 * s + 1]), im, false);
+      re = __builtin_amdgcn_fdot2(as_hv2(u2), as_hv2(hb[3 * s + 2]), re, false);
+      im = __builtin_amdgcn_fdot2(as_hv2(u2), as_hv2(hx[3 * s + 2]), im, false);
+      w[3 * s + r] = cmk<float>(re, im);
+    }
+  reconstruct_scaled<MU, A>(acc, w, sr.n);
+}
+
+template <int D, typename LK, int DAG>
+__device__ __forceinline__ void dir_compute_h(const SpinRaw<SpinH>& sr, const LinkRaw<LK>& lr, float sign,
+                                              cplx<float> acc[12]) {
+  constexpr int MU = D & 3;
+  constexpr int S = (DAG == 0) ? 1 : -1;
+  constexpr int A = (D < 4) ? -S : S;
+  hv2 h[6];
+  project_h<MU, A>(sr, h);
+  uint32_t U[9];
+  link_words<LK>(lr, sign, U);
+  cplx<float> w[6];
+  if constexpr (LinkIsBf16<LK>::value) {
+    bv2 hb[6], hx[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const float2 f = make_float2((float)h[k].x, (float)h[k].y);
+      if constexpr (D < 4) {   // U h: re = (Ur,Ui).(hr,-hi), im = (Ur,Ui).(hi,hr)
+        hb[k] = (bv2){(__bf16)f.x, (__bf16)(-f.y)};
+        hx[k] = (bv2){(__bf16)f.y, (__bf16)f.x};
+      } else {                 // U^+ h: re = (Ur,Ui).(hr,hi), im = (Ur,Ui).(hi,-hr)
+        hb[k] = (bv2){(__bf16)f.x, (__bf16)f.y};
+        hx[k] = (bv2){(__bf16)f.y, (__bf16)(-f.x)};
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        float re = 0.f, im = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const bv2 u = as_bv2(D < 4 ? U[3 * r + c] : U[3 * c + r]);
+          re = __builtin_amdgcn_fdot2_f32_bf16(u, hb[3 * s + c], re, false);
+          im = __builtin_amdgcn_fdot2_f32_bf16(u, hx[3 * s + c], im, false);
+        }
+        w[3 * s + r] = cmk<float>(re, im);
+      }
+  } else {
+    hv2 hb[6], hx[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      if constexpr (D < 4) {
+        hb[k] = h[k] * (hv2){(_Float16)1, (_Float16)(-1)};
+        hx[k] = hswap(h[k]);
+      } else {
+        hb[k] = h[k];
+        hx[k] = hswap(h[k]) * (hv2){(_Float16)1, (_Float16)(-1)};
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int r = 0; r < 3; r++) {
+        float re = 0.f, im = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const hv2 u = as_hv2(D < 4 ? U[3 * r + c] : U[3 * c + r]);
+          re = __builtin_amdgcn_fdot2(u, hb[3 * s + c], re, false);
+          im = __builtin_amdgcn_fdot2(u, hx[3 * s + c], im, false);
+        }
+        w[3 * s + r] = cmk<float>(re, im);
+      }
+  }
+  reconstruct_scaled<MU, A>(acc, w, sr.n);
+}
+
+// forward:  U_mu(x) (1 - s gamma_mu) psi(x+mu)        (link of the output site)
+// backward: U_mu(x-mu)^dagger (1 + s gamma_mu) psi(x-mu) (link of the neighbour)
+template <int D, typename SIN, typename LK, typename R, int DAG>
+__device__ __forceinline__ void dir_compute(const SpinRaw<SIN>& sr, const LinkRaw<LK>& lr, R sign, cplx<R> acc[12]) {
+  if constexpr (std::is_same<SIN, SpinH>::value) {
+    dir_compute_h2<D, LK, DAG>(sr, lr, float(sign), acc);
+    return;
+  }
+  constexpr int MU = D & 3;
+  constexpr int S = (DAG == 0) ? 1 : -1;
+  cplx<R> psi[12], h[6], w[6], u[9];
+  raw_decode<SIN, R>(sr, psi);
+  link_decode<LK, R>(lr, sign, u);
+  if constexpr (D < 4) {
+    project<MU, -S, R>(psi, h);
+    su3_mul<R>(u, h, w);
+    reconstruct<MU, -S, R>(acc, w);
+  } else {
+    project<MU, S, R>(psi, h);
+    su3_dag_mul<R>(u, h, w);
+    reconstruct<MU, S, R>(acc, w);
+  }
+}
+
+// Directions D0 .. D0+NB-1: all their spinor and link loads are issued
+// before the first of them is decoded.
+template <int D0, int NB, typename SIN, typename LK, typename R, int DAG>
+__device__ __forceinline__ void hop_batch(const HopParams& P, const Nbr& N, int out_par, int site_cb,
+                                          int64_t in_base, int j, cplx<R> acc[12]) {
+  const int Vcb = P.geom.Vcb;
+  SpinRaw<SIN> sr[NB];
+  LinkRaw<LK> lr[NB];
+#pragma unroll
+  for (int k = 0; k < NB; k++) {
+    const int d = D0 + k;
+    raw_load<SIN>(sr[k], P.in, in_base + (int64_t)N.idx[d] * P.nb + j);
+    const int mu = d & 3;
+    const int64_t lidx = (d < 4) ? (int64_t)(out_par * 4 + mu) * Vcb + site_cb
+                                 : (int64_t)((1 - out_par) * 4 + mu) * Vcb + N.idx[d];
+    link_raw_load<LK>(lr[k], P.gauge, lidx);
+  }
+#define CPSHIP_DIR(K)                                                                             \
+  if constexpr ((K) < NB) {                                                                       \
+    const R sg = (P.link_bc_sign && ((N.neg >> (D0 + (K))) & 1u)) ? R(-1) : R(1);                 \
+    dir_compute<D0 + (K), SIN, LK, R, DAG>(sr[K], lr[K], sg, acc);                                \
+  }
+  CPSHIP_DIR(0) CPSHIP_DIR(1) CPSHIP_DIR(2) CPSHIP_DIR(3) CPSHIP_DIR(4) CPSHIP_DIR(5) CPSHIP_DIR(6) CPSHIP_DIR(7)
+#undef CPSHIP_DIR
+}
+
+template <typename R>
+__device__ __forceinline__ R wave_sum(R v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum v over the block (all threads of a block share one rhs) and write one
+// partial per block.
+template <typename R>
+__device__ __forceinline__ void block_partial(R v, double* partial, int64_t slot) {
+  __shared__ double red[16];
+  double d = wave_sum<double>(double(v));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int k = 0; k < nw; k++) s += red[k];
+    partial[slot] = s;
+  }
+}
+
+// Maps the hardware block id so that each XCD (blocks are dealt round-robin
+// over the 8 XCDs) works on a contiguous range of sites, which keeps
+// neighbour reuse inside one XCD's L2.
+__device__ __forceinline__ int64_t logical_block(int64_t bid, int64_t nblk, int remap) {
+  if (!remap || (nblk & 7) != 0) return bid;
+  return (bid & 7) * (nblk >> 3) + (bid >> 3);
+}
+
+// Register budget of the hop kernels: at least this many waves per SIMD
+// (512 VGPRs / waves).  With the load batch depth (HopBatch) it trades loads
+// in flight per wave against waves per SIMD; the defaults are the measured
+// best on MI355X (half: batch 4, 3 waves, 47 us vs 56 us for batch 8 with 2
+// waves on the 16^3x32x16 domain-wall hop; FP64: batch 2, 2 waves).
+#ifndef CPSHIP_HOP_WAVES_D
+#define CPSHIP_HOP_WAVES_D 2
+#endif
+#ifndef CPSHIP_HOP_WAVES_F
+#define CPSHIP_HOP_WAVES_F 3
+#endif
+#ifndef CPSHIP_HOP_WAVES_H
+#define CPSHIP_HOP_WAVES_H 3
+#endif
+template <typename S> struct HopWaves;
+template <> struct HopWaves<SpinD> { static constexpr int value = CPSHIP_HOP_WAVES_D; };
+template <> struct HopWaves<SpinF> { static constexpr int value = CPSHIP_HOP_WAVES_F; };
+template <> struct HopWaves<SpinH> { static constexpr int value = CPSHIP_HOP_WAVES_H; };
+
+template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV>
+__global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) {
+  const int64_t nblk = (int64_t)gridDim.x;
+  const int64_t lb = logical_block(blockIdx.x, nblk, !(P.flags & HOP_NO_XCD_REMAP));
+  const int64_t e = lb * blockDim.x + threadIdx.x;   // element within this launch
+  const int64_t per_rhs = (int64_t)P.nsites_out * P.nb;
+  const int rhs = (int)(e / per_rhs);
+  const int64_t rem = e - (int64_t)rhs * per_rhs;
+  const int site = (int)(rem / P.nb);
+  const int j = (int)(rem - (int64_t)site * P.nb);
+  const bool valid = rhs < P.nrhs;
+  R nrm = R(0);
+  if (valid) {
+    const Geom& g = P.geom;
+    int out_par, site_cb;
+    if (KIND == KIND_DWF) {       // fields hold both 4-D parities
+      out_par = site >= g.Vcb ? 1 : 0;
+      site_cb = site - out_par * g.Vcb;
+    } else {
+      out_par = P.out_parity;
+      site_cb = site;
+    }
+
+    int64_t in_base;
+    if (KIND == KIND_DWF) in_base = ((int64_t)rhs * 2 * g.Vcb + (int64_t)(1 - out_par) * g.Vcb) * P.nb;
+    else in_base = (int64_t)rhs * g.Vcb * P.nb;
+
+    cplx<R> acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = cmk<R>(R(0), R(0));
+    // neighbour indices and boundary bits of this site, precomputed per
+    // (parity, site) by k_build_nbr (the j lanes of a site read one address)
+    Nbr N;
+    {
+      const int64_t ns = (int64_t)out_par * g.Vcb + site_cb;
+      const uint4 a = P.nbr[ns], b = P.nbr[2 * (int64_t)g.Vcb + ns];
+      N.idx[0] = (int)a.x; N.idx[1] = (int)a.y; N.idx[2] = (int)a.z; N.idx[3] = (int)a.w;
+      N.idx[4] = (int)b.x; N.idx[5] = (int)b.y; N.idx[6] = (int)b.z; N.idx[7] = (int)b.w;
+      N.neg = P.nbr_neg[ns];
+    }
+    constexpr int NB = HopBatch<SIN>::value;
+    if constexpr (NB == 8) {
+      hop_batch<0, 8, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+    } else if constexpr (NB == 4) {
+      hop_batch<0, 4, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<4, 4, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+    } else if constexpr (NB == 2) {
+      hop_batch<0, 2, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<2, 2, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<4, 2, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<6, 2, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+    } else {
+      hop_batch<0, 1, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<1, 1, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<2, 1, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<3, 1, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<4, 1, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<5, 1, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<6, 1, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+      hop_batch<7, 1, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
+    }
+
+    if constexpr (KIND == KIND_DWF) {
+      // fifth dimension: input slots at the same 4-D site (same parity block)
+      const int ls = P.ls;
+      const int cls_out = (out_par + P.out_cb5) & 1;        // s = 2j + cls_out
+      const int s = 2 * j + cls_out;
+      const int64_t site_base = ((int64_t)rhs * 2 * g.Vcb + (int64_t)out_par * g.Vcb + site_cb) * P.nb;
+      // in slots at this site hold s' = 2j' + (1 - cls_out)
+      const int sm = (s == 0) ? ls - 1 : s - 1;
+      const int sp = (s == ls - 1) ? 0 : s + 1;
+      const R cm = (s == 0) ? R(P.neg_mass_two_over_a5) : R(P.two_over_a5);
+      const R cp = (s == ls - 1) ? R(P.neg_mass_two_over_a5) : R(P.two_over_a5);
+      const int jm = sm >> 1, jp = sp >> 1;
+      // dag 0: upper from s-1, lower from s+1; dag 1: lower from s-1, upper from s+1
+      const int c_from_m = (DAG == 0) ? 0 : 6;
+      const int c_from_p = (DAG == 0) ? 6 : 0;
+      cplx<R> hm[6], hp[6];
+      spin_load_half<SIN, R>(P.in, site_base + jm, c_from_m, hm);
+      spin_load_half<SIN, R>(P.in, site_base + jp, c_from_p, hp);
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        acc[c_from_m + k].re += cm * hm[k].re;
+        acc[c_from_m + k].im += cm * hm[k].im;
+        acc[c_from_p + k].re += cp * hp[k].re;
+        acc[c_from_p + k].im += cp * hp[k].im;
+      }
+    }
+
+    const int64_t eo = (int64_t)rhs * per_rhs + rem;
+    cplx<R> res[12];
+    if constexpr (CLOV != CLOV_NONE) {
+      const int set = rhs / P.rhs_per_set;
+      // (no branch here: a conditional load splits the block and the stencil
+      // above it then spills)
+      R sc;
+      if constexpr (CLOV == CLOV_X) sc = R(P.scale * P.kscale[set]);
+      else sc = R(P.scale);
+      const int64_t c0 = (int64_t)(set * 2 + out_par) * CloverOf<SIN>::nplanes * P.clover.plane + site_cb;
+      if constexpr (CLOV == CLOV_POST) {
+#pragma unroll
+        for (int k = 0; k < 12; k++) acc[k] = cmk<R>(sc * acc[k].re, sc * acc[k].im);
+        clover_mul<SIN, R>(P.clover, c0, acc, res);
+      } else {
+        cplx<R> xv[12], ax[12];
+        spin_load<SOUT, R>(P.X, eo, xv);
+        clover_mul<SIN, R>(P.clover, c0, xv, ax);
+        const R xc = R(P.xcoef);
+#pragma unroll
+        for (int k = 0; k < 12; k++)
+          res[k] = cmk<R>(xc * ax[k].re + sc * acc[k].re, xc * ax[k].im + sc * acc[k].im);
+      }
+    } else {
+      const R sc = R(P.scale);
+      if (P.X.data) {
+        cplx<R> xv[12];
+        spin_load<SOUT, R>(P.X, eo, xv);
+        const R xc = R(P.xcoef);
+#pragma unroll
+        for (int k = 0; k < 12; k++) res[k] = cmk<R>(xc * xv[k].re + sc * acc[k].re, xc * xv[k].im + sc * acc[k].im);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 12; k++) res[k] = cmk<R>(sc * acc[k].re, sc * acc[k].im);
+      }
+    }
+    spin_store<SOUT, R, NT>(P.out, eo, res);
+    if (P.partial) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) nrm += res[k].re * res[k].re + res[k].im * res[k].im;
+    }
+  }
+  if (P.partial) {
+    // blocks never straddle two rhs (block size divides nsites_out*nb)
+    const int64_t bpr = per_rhs / blockDim.x;
+    block_partial<R>(nrm, P.partial, lb);
+    (void)bpr;
+  }
+}
+
+}  // namespace cpship

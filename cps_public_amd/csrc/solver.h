@@ -44,11 +44,11 @@ struct OpLayout {
 };
 
 struct HopTuneKey {
-  int kind, prec, dag, nb, nsites, nrhs, partial, hasX;
+  int kind, prec, dag, nb, nsites, nrhs, partial, hasX, clov;
   bool operator<(const HopTuneKey& o) const {
-    const int a[8] = {kind, prec, dag, nb, nsites, nrhs, partial, hasX};
-    const int b[8] = {o.kind, o.prec, o.dag, o.nb, o.nsites, o.nrhs, o.partial, o.hasX};
-    for (int k = 0; k < 8; k++)
+    const int a[9] = {kind, prec, dag, nb, nsites, nrhs, partial, hasX, clov};
+    const int b[9] = {o.kind, o.prec, o.dag, o.nb, o.nsites, o.nrhs, o.partial, o.hasX, o.clov};
+    for (int k = 0; k < 9; k++)
       if (a[k] != b[k]) return a[k] < b[k];
     return false;
   }
@@ -70,9 +70,12 @@ struct Lattice {
   long long gauge_generation = 0;
   std::vector<Field> scratch;
   DevBuf partial, scal, io, io2;
-  // clover: per kappa, A_oo and A_ee^-1 packed (72 doubles per site) for
-  // each checkerboard, layout [k][cb][site][72]
-  DevBuf clover;
+  DevBuf nbr, nbr_neg;   // neighbour table of the geometry (hop kernels)
+  // clover sets [kappa][cb] (cb 0: A_ee^-1, cb 1: A_oo) per storage format;
+  // FP64 is built from the gauge, FP32/FP16 are converted from it on demand
+  DevBuf clover[NCLOVER];
+  bool clover_valid[NCLOVER];
+  DevBuf clover_ksq;   // kappa^2 per set (device)
   int n_clover = 0;
   std::vector<double> clover_kappas;
   double clover_csw = 0;
@@ -82,11 +85,59 @@ struct Lattice {
   ~Lattice();
   int init(const int node_sites[4], int ls, const int bc[4]);
   int import_gauge(const double* u_host);
+  int import_gauge_device(const double* u_dev);   // same, from device memory
+  int import_gauge_from(const double* u, hipMemcpyKind kind);
   int ensure_gauge(int link_storage);
   GaugeRef gauge_ref(int link_storage) const;
   Field& field(int slot, int storage, int64_t nelem);
   int ensure_partial(int64_t slots);
   long long device_bytes() const;
+};
+
+// One CG iteration's launch sequence replayed as a hipGraph.  The first
+// iteration runs eagerly (the launch autotuner times its candidates there,
+// which cannot happen inside a capture); the same sequence is then captured
+// once and every later iteration is one graph launch.  Only kernel launches
+// are captured (no memset nodes).  All kernel arguments are device pointers
+// or values fixed for the solve, so one capture serves every iteration.
+struct IterGraph {
+  bool enabled;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  explicit IterGraph(bool on) : enabled(on) {}
+  IterGraph(const IterGraph&) = delete;
+  IterGraph& operator=(const IterGraph&) = delete;
+  ~IterGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  template <class F>
+  int run(Lattice& L, int n, F&& body) {
+    int rc = 0;
+    if (n <= 0) return 0;
+    if (!enabled) {
+      for (int i = 0; i < n; i++)
+        if ((rc = body())) return rc;
+      return 0;
+    }
+    if (!exec) {
+      if ((rc = body())) return rc;
+      n--;
+      CPSHIP_CHECK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+      rc = body();
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(L.stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (ec != hipSuccess) return CPS_HIP_ERR_RUNTIME;
+      graph = g;
+      CPSHIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < n; i++) CPSHIP_CHECK(hipGraphLaunch(exec, L.stream));
+    return 0;
+  }
 };
 
 struct CGState {
@@ -102,6 +153,15 @@ int op_layout(const cps_hip_op* op, int ls, OpLayout* o, const Geom& g);
 double op_kappa(const cps_hip_op* op);
 int hop(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
         double scale, const FieldRef& X, double xcoef, double* partial, int nrhs);
+// clover epilogue of a Wilson hop (HopParams::clov)
+struct HopClover {
+  int mode;
+  CloverRef ref;
+  const double* kscale;
+  int rhs_per_set;
+};
+int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
+           double scale, const FieldRef& X, double xcoef, double* partial, int nrhs, const HopClover* hc);
 int hop_blocks_per_rhs(Lattice& L, const cps_hip_op* op);
 int tuned_launch(Lattice& L, HopParams& P);
 void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp_store, int* hprec);
@@ -122,6 +182,7 @@ int clover_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& ou
 int clover_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
                  const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs);
 int clover_build(Lattice& L, const double* kappas, int n_kappa, double csw);
+int clover_export(Lattice& L, int k, int cb, double* A);
 
 // host <-> device field transfer with layout conversion; with on_device the
 // double pointer is device memory in the CPS layout (no PCIe copy)

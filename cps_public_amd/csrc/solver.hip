@@ -58,6 +58,7 @@ FieldRef null_field() {
 // ---------------------------------------------------------------------------
 Lattice::Lattice() {
   for (int k = 0; k < NLINK; k++) gauge_valid[k] = false;
+  for (int k = 0; k < NCLOVER; k++) clover_valid[k] = false;
   // every slot exists up front: references handed out by field() stay valid
   scratch.resize(S_NSLOTS);
 }
@@ -81,23 +82,36 @@ int Lattice::init(const int node_sites[4], int ls_, const int bc[4]) {
     return CPS_HIP_ERR_NODEV;
   }
   if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return CPS_HIP_ERR_RUNTIME;
+  int rc = nbr.ensure((size_t)4 * g.Vcb * sizeof(uint4));
+  if (rc) return rc;
+  if ((rc = nbr_neg.ensure((size_t)2 * g.Vcb * sizeof(uint32_t)))) return rc;
+  CPSHIP_CHECK(build_nbr(g, (uint4*)nbr.p, (uint32_t*)nbr_neg.p, stream));
+  CPSHIP_CHECK(hipStreamSynchronize(stream));
   return 0;
 }
 
 long long Lattice::device_bytes() const {
-  long long b = (long long)gauge_aos.bytes + partial.bytes + scal.bytes + io.bytes + io2.bytes;
+  long long b = (long long)gauge_aos.bytes + partial.bytes + scal.bytes + io.bytes + io2.bytes + nbr.bytes +
+                nbr_neg.bytes;
   for (int k = 0; k < NLINK; k++) b += gauge[k].bytes;
   for (auto& f : scratch) b += f.buf.bytes + f.nrm.bytes;
-  b += clover.bytes;
+  for (int k = 0; k < NCLOVER; k++) b += clover[k].bytes;
   return b;
 }
 
-int Lattice::import_gauge(const double* u_host) {
+int Lattice::import_gauge(const double* u_host) { return import_gauge_from(u_host, hipMemcpyHostToDevice); }
+
+int Lattice::import_gauge_device(const double* u_dev) { return import_gauge_from(u_dev, hipMemcpyDeviceToDevice); }
+
+int Lattice::import_gauge_from(const double* u, hipMemcpyKind kind) {
   const size_t n = (size_t)72 * 2 * g.Vcb;
   int rc = gauge_aos.ensure(n * sizeof(double));
   if (rc) return rc;
-  CPSHIP_CHECK(hipMemcpyAsync(gauge_aos.p, u_host, n * sizeof(double), hipMemcpyHostToDevice, stream));
+  CPSHIP_CHECK(hipMemcpyAsync(gauge_aos.p, u, n * sizeof(double), kind, stream));
   for (int k = 0; k < NLINK; k++) gauge_valid[k] = false;
+  // a clover field built from the previous gauge field is stale
+  n_clover = 0;
+  for (int k = 0; k < NCLOVER; k++) clover_valid[k] = false;
   rc = ensure_gauge(LINK_D18);
   if (rc) return rc;
   CPSHIP_CHECK(hipStreamSynchronize(stream));
@@ -185,6 +199,11 @@ double op_kappa(const cps_hip_op* op) {
 // output lives on `target` (Wilson: 4-D parity; DWF: 5-D checkerboard).
 int hop(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
         double scale, const FieldRef& X, double xcoef, double* partial, int nrhs) {
+  return hop_ex(L, op, hprec, out, in, target, dag, scale, X, xcoef, partial, nrhs, nullptr);
+}
+
+int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
+           double scale, const FieldRef& X, double xcoef, double* partial, int nrhs, const HopClover* hc) {
   OpLayout o;
   int rc = op_layout(op, L.ls, &o, L.g);
   if (rc) return rc;
@@ -214,6 +233,12 @@ int hop(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const 
   if (rc) return rc;
   P.gauge = L.gauge_ref(lstore);
   P.partial = partial;
+  if (hc) {
+    P.clov = hc->mode;
+    P.clover = hc->ref;
+    P.kscale = hc->kscale;
+    P.rhs_per_set = hc->rhs_per_set;
+  }
   rc = tuned_launch(L, P);
   return rc;
 }
@@ -236,9 +261,12 @@ static bool tuning_enabled() {
 }
 
 int tuned_launch(Lattice& L, HopParams& P) {
+  P.nbr = (const uint4*)L.nbr.p;
+  P.nbr_neg = (const uint32_t*)L.nbr_neg.p;
   const long long per_rhs = (long long)P.nsites_out * P.nb;
   const int def_block = pick_block(per_rhs);
-  HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr};
+  HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr,
+                 P.clov};
   auto it = L.hop_tune.find(key);
   if (it == L.hop_tune.end()) {
     HopTuneVal best{def_block, 0, 0.0};
@@ -395,6 +423,36 @@ static bool verbose() {
   return v == 1;
 }
 
+// CPS_HIP_PROGRESS=<seconds> prints one line to stderr at most every that
+// many seconds while a solve runs (iteration, rhs still active, elapsed).
+struct Progress {
+  double every = 0.0;
+  std::chrono::steady_clock::time_point t0, last;
+  const char* what;
+  explicit Progress(const char* w) : what(w) {
+    const char* e = getenv("CPS_HIP_PROGRESS");
+    every = e ? atof(e) : 0.0;
+    t0 = last = std::chrono::steady_clock::now();
+  }
+  // rr: device |r|^2 per rhs (the largest is printed)
+  void tick(Lattice& L, int iter, const std::vector<int>& act, const double* rr) {
+    if (every <= 0.0) return;
+    const auto now = std::chrono::steady_clock::now();
+    if (std::chrono::duration<double>(now - last).count() < every) return;
+    last = now;
+    int n = 0;
+    for (int a : act) n += a != 0;
+    std::vector<double> h(act.size(), 0.0);
+    double mx = 0.0;
+    if (hipMemcpyAsync(h.data(), rr, h.size() * sizeof(double), hipMemcpyDeviceToHost, L.stream) == hipSuccess &&
+        hipStreamSynchronize(L.stream) == hipSuccess)
+      for (double v : h) mx = std::max(mx, v);
+    fprintf(stderr, "cps_hip %s CG: iteration %d, %d of %zu rhs active, max |r|^2 %.3e, %.1f s\n", what, iter, n,
+            act.size(), mx, std::chrono::duration<double>(now - t0).count());
+    fflush(stderr);
+  }
+};
+
 static int sync_read(Lattice& L, void* dst, const void* src, size_t bytes) {
   CPSHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, L.stream));
   CPSHIP_CHECK(hipStreamSynchronize(L.stream));
@@ -456,17 +514,22 @@ int cg_double_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int
       if (act[k]) return true;
     return false;
   };
+  Progress prog("fp64");
+  IterGraph iter_graph(arg->use_graph != 0);
   while (any_active() && done_iters < max_itr) {
+    prog.tick(L, done_iters, act, cg.s.rr);
     const int todo = std::min(chunk, max_itr - done_iters);
-    for (int it = 0; it < todo; it++) {
+    rc = iter_graph.run(L, todo, [&]() -> int {
       // mmp = MdagM p, d = |M p|^2 (inv_cg.C:302)
-      rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), p.ref(), t1.ref(), t2.ref(), part, nrhs);
-      if (rc) return rc;
+      int e = apply_mdagm(L, op, PREC_D_D18, ap.ref(), p.ref(), t1.ref(), t2.ref(), part, nrhs);
+      if (e) return e;
       CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nrhs, nullptr, RED_CG_ALPHA, cg.s, L.stream));
       CPSHIP_CHECK(blas_cg_xr_d(x.ref(), p.ref(), r.ref(), ap.ref(), cg.s, o.per_rhs, nrhs, bsb, part, L.stream));
       CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_CG_BETA, cg.s, L.stream));
       CPSHIP_CHECK(blas_xpay(STORE_D, p.ref(), r.ref(), cg.s.beta, cg.s.active, o.per_rhs, nrhs, bsb, L.stream));
-    }
+      return 0;
+    });
+    if (rc) return rc;
     done_iters += todo;
     rc = sync_read(L, act.data(), cg.s.active, nrhs * sizeof(int));
     if (rc) return rc;
@@ -628,18 +691,23 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
       if (v[k]) return true;
     return false;
   };
+  Progress prog("mixed");
+  IterGraph iter_graph(arg->use_graph != 0);
   while (any(live) && done < max_itr) {
+    prog.tick(L, done, live, cg.s.rr);
     const int todo = std::min(std::min(chunk, max_itr - done), every - since);
-    for (int it = 0; it < todo; it++) {
-      rc = apply_mdagm(L, op, hprec, aps.ref(), ps.ref(), t1s.ref(), t2s.ref(), part, nrhs);
-      if (rc) return rc;
+    rc = iter_graph.run(L, todo, [&]() -> int {
+      int e = apply_mdagm(L, op, hprec, aps.ref(), ps.ref(), t1s.ref(), t2s.ref(), part, nrhs);
+      if (e) return e;
       CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nrhs, nullptr, RED_CG_ALPHA, cg.s, L.stream));
       CPSHIP_CHECK(blas_cg_xr_mixed(STORE_F, xs.ref(), sps, ps.ref(), STORE_F, rs.ref(), sps, aps.ref(), cg.s,
                                     o.per_rhs, nrhs, bsb, part, L.stream));
       CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_CG_BETA, cg.s, L.stream));
       CPSHIP_CHECK(blas_xpay_mixed(sps, ps.ref(), STORE_F, rs.ref(), cg.s.beta, cg.s.active, o.per_rhs, nrhs, bsb,
                                    L.stream));
-    }
+      return 0;
+    });
+    if (rc) return rc;
     done += todo;
     since += todo;
     rc = sync_read(L, act.data(), cg.s.active, nrhs * sizeof(int));

@@ -77,6 +77,7 @@ typedef struct {
   int reliable_every;    /* mixed: FP64 reliable update at least every N iterations */
   double reliable_delta; /* mixed: also when |r_sloppy| < delta * |r| at last update */
   int check_every;       /* host convergence polling interval (iterations), 0 = auto */
+  int use_graph;         /* 1: replay the CG iteration as a captured hipGraph, 0: eager launches */
 } cps_hip_cg_arg;
 
 /* Per-rhs result of a solve. */
@@ -183,6 +184,39 @@ int cps_hip_bench_hop(cps_hip_lattice *lat, int kind, int precision, int nrhs, i
  * field) of n_elem spinors (192 bytes read + 192 bytes written each): the
  * known-size reference that scales hardware traffic counters. */
 int cps_hip_bench_copy(cps_hip_lattice *lat, long long n_elem, int reps, double *seconds);
+
+/* ---- resident solve service -----------------------------------------------
+ * Keeps gauge fields on the device and serves propagator solves by
+ * continuous batching: every (lattice, operator) pair in use gets an engine,
+ * a batched FP64 CG (inv_cg.C:293-364 per slot) over max_slots right-hand
+ * side slots on its own stream; a slot whose rhs converged is harvested and
+ * refilled with the next waiting rhs while the others iterate.  Engines of
+ * different lattices/masses run concurrently; an engine is created only while
+ * the estimated device footprint fits hbm_budget_bytes (idle engines are
+ * released first).  submit() returns at once; src and sol (host, CPS layout,
+ * nrhs vectors) must stay valid until wait() returns.  No counterpart in CPS. */
+typedef struct cps_hip_service cps_hip_service;
+typedef struct {
+  long long requests_done;
+  long long rhs_done;
+  long long iterations;    /* CG iterations summed over engines */
+  long long peak_bytes;    /* peak device footprint of gauges + engines */
+  int engines_created;
+  int engines_live;
+  int last_error;
+} cps_hip_service_stats;
+int cps_hip_service_create(cps_hip_service **s, long long hbm_budget_bytes, int max_slots);
+int cps_hip_service_destroy(cps_hip_service *s);
+/* Registers a gauge field (WILSON order, boundary signs applied) for lattices
+ * of the given extent and Ls; copies it to the device. */
+int cps_hip_service_add_gauge(cps_hip_service *s, const int node_sites[4], int ls, const int bc[4],
+                              const double *u, int *gauge_id);
+int cps_hip_service_submit(cps_hip_service *s, int gauge_id, const cps_hip_op *op, const double *src, double *sol,
+                           int nrhs, double stop_rsd, int max_num_iter, long long *request_id);
+/* Blocks until the request is done; res has nrhs entries; latency_s is the
+ * time from submit to the last rhs harvested. */
+int cps_hip_service_wait(cps_hip_service *s, long long request_id, cps_hip_cg_result *res, double *latency_s);
+int cps_hip_service_get_stats(cps_hip_service *s, cps_hip_service_stats *st);
 
 /* ---- device memory for device-resident callers ---------------------------
  * Plain device buffers on the library's own HIP runtime, for callers that

@@ -34,8 +34,9 @@ static void axpy(double *out, double z, const double *in, int n) {
 
 /* zmobius_kappa_dslash_5_plus_cmplx, zmobius_dslash_5_plus.C:125-391 with a
  * constant real factor fact[s] = coef (s_nodes = 1).  Accumulates into out.
- * Pinned by: test_oracle_mobius (b=1,c=0 reduction to Shamir DWF and the
- * independently built free-field 5-D operator). */
+ * Pinned by: tests/test_oracle_pins.py::test_mobius_preconditioned_ops (the
+ * Schur complement of an independently built sparse Moebius operator, b,c =
+ * (1.5,0.5), (1,0), (2,1)) and ::test_mobius_m5inv. */
 void ora_mobius_dslash_5_plus(double *out, const double *in, double mass,
                               int dag, double coef, const ora_mobius *m) {
   int ls = m->ls, vol_4d_cb = vol4(m->L) / 2;
@@ -83,8 +84,8 @@ void ora_mobius_dslash_4(double *out, const double *u, const double *in,
 
 /* zmobius_m5inv_cmplx_dag0 / _dag1, zmobius_m5inv.C:247-686, for a constant
  * real K = kappa_b/kappa_c.  In place.
- * Pinned by: test_oracle_mobius (M5inv applied after the M5 it inverts gives
- * the identity). */
+ * Pinned by: tests/test_oracle_pins.py::test_mobius_m5inv (M5 applied to the
+ * result, with M5 built independently, gives the input back; even Ls). */
 void ora_mobius_m5inv(double *f, double mass, int dag, const ora_mobius *m) {
   int ls = m->ls, vol = vol4(m->L) / 2;
   double K, kb, kc, prod, inv_d_last, factL, factR;
@@ -243,9 +244,11 @@ int ora_mobius_invcg(double *sol, const double *src, const double *u,
  * formed by an LDL^dagger factorisation of the packed Hermitian matrix as
  * mat_inv(MAT_INV_ALG_LDL_CMPR) does (dense_matrix.C:188-262); the rounding
  * of that step is this restatement's own.
- * Pinned by: test_oracle_clover (c_sw = 0 reduces to Wilson, Hermiticity of
- * the packed blocks, A A^{-1} = 1, and the field strength of an abelian
- * constant-field configuration against its closed form).
+ * Pinned by: tests/test_oracle_pins.py::test_clover_matrix_textbook (every
+ * site's 12x12 matrix against the documented formula of dirac_op.h:930-951
+ * built independently from the four leaves, A A^{-1} = 1) and
+ * ::test_clover_preconditioned_ops (Schur complement of A - kappa H, c_sw = 0
+ * reduces to Wilson).
  * ====================================================================== */
 typedef struct { double re, im; } cplx;
 typedef struct { cplx e[3][3]; } mat3;

@@ -22,8 +22,9 @@ static int vol4(const int L[4]) { return L[0] * L[1] * L[2] * L[3]; }
  * site number (x - x%2 + lx*(y+ly*(z+lz*t)) + vol*parity)/2 and the link
  * reals moved from r+2*(col+3*(row+3*mu)) to r+2*(row+3*(col+3*mu))), and
  * convert_func.C:429-494 (fermion, odd sites first).
- * Pinned by: test_oracle_layout (round trips, and Dslash in WILSON order
- * equal to the dense CANONICAL operator built independently in numpy).
+ * Pinned by: tests/test_oracle_pins.py::test_fermion_layouts,
+ * ::test_gauge_layout, ::test_bond_cond (the index formulas restated
+ * independently, round trips).
  * ====================================================================== */
 static int wilson_gauge_site(const int L[4], int x, int y, int z, int t) {
   int vol = vol4(L);
@@ -187,10 +188,11 @@ void ora_bond_cond(double *u, const int L[4], const int bc[4]) {
  * OneMinusPlusGammaTimesPsi and the link multiply UmuTimes, kept as full
  * 4-spinor operations as CPS does) and wilson_dslash.C:173-515 (site loop,
  * neighbour addressing, link offsets, order of the 8-term sum).
- * Pinned by: test_oracle_wilson (Clifford algebra and gamma_5 of the
- * extracted projectors against the matrices documented in dirac_op.h,
- * free-field plane-wave eigenvalues, gauge covariance, gamma_5
- * hermiticity, and equality with an independently built dense operator).
+ * Pinned by: tests/test_oracle_pins.py::test_wilson_dslash_textbook (both
+ * cb and dag against an independently built sparse hopping matrix with the
+ * gamma basis of dirac_op.h:955-978, dag 1 as its adjoint),
+ * ::test_wilson_plane_wave (closed form), ::test_wilson_gauge_covariance,
+ * ::test_wilson_preconditioned_ops (Schur complement).
  * ====================================================================== */
 #define UELEM(u, r, row, col, d) (u)[(r) + 2 * ((row) + 3 * ((col) + 3 * (d)))]
 #define FERM(f, r, c, s) (f)[(r) + 2 * ((c) + 3 * (s))]
@@ -368,9 +370,10 @@ void ora_ftimesv1plusv2(double *a, double b, const double *c, const double *d,
  * Shamir domain wall.  dwf_init.C:97-98 (kappa), dwf_dslash.C:32-50,
  * dwf_dslash_4.C:68-141, dwf_dslash_5_plus.C:111-340 (s_nodes = 1),
  * dwf_m.C:31-62, dwf_mdag.C:32-62, dwf_mdagm.C:32-61.
- * Pinned by: test_oracle_dwf (free-field 5-D momentum-space operator built
- * independently, Hermiticity / positivity of MdagM, gamma_5 R_5
- * hermiticity of the unpreconditioned operator, brute-force solve).
+ * Pinned by: tests/test_oracle_pins.py::test_dwf_dslash_textbook (both cb
+ * and dag against an independently built sparse 5-D operator) and
+ * ::test_dwf_preconditioned_ops_and_solvers (Schur complement, kappa_5,
+ * CG and MatInv residuals through the sparse operator).
  * ====================================================================== */
 double ora_dwf_kappa(const ora_dwf *d) {
   return 1.0 / (2 * (4 + d->a5_inv - d->m5));
@@ -474,9 +477,11 @@ void ora_dwf_mdagm(double *out, const double *u, const double *in,
  * res_hist (optional, max_num_iter entries) receives |res|^2 after each
  * iteration.  d == 0 makes the reference exit(5) (inv_cg.C:326-329); here it
  * returns -1 so a test can observe it.
- * Pinned by: test_oracle_cg (brute-force dense solve on small lattices, the
- * CPS self-check max|Mx-b| of tests/f_wilson_inv/main.C:183-246, and the
- * iteration scale logged in tests/f_wilson_inv/qcdlinux1.log:69).
+ * Pinned by: tests/test_oracle_pins.py::test_wilson_matinv_and_cg (the CPS
+ * self-check max|Mx-b| of tests/f_wilson_inv/main.C:183-246 through the
+ * independently built operator), ::test_wilson_cg_iteration_scale (the
+ * iteration count logged in tests/f_wilson_inv/qcdlinux1.log:69), and the
+ * solver residual checks of the DWF, Moebius and clover tests.
  * ====================================================================== */
 int ora_invcg(double *sol, const double *src, size_t f_size_cb, double stop_rsd,
               int max_num_iter, double src_norm_sq, double *true_res,

@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
 same seeded inputs.  Tolerances: FP64 operators 1e-13 relative (rounding of a
 different summation order only); FP64 CG solutions 1e-10 relative (the
-north-star bound); mixed-precision CG true residual <= stop_rsd, solution
+bound the solver is specified to); mixed-precision CG true residual <= stop_rsd, solution
 within 1e-8 of the FP64 oracle."""
 import numpy as np
 import pytest
@@ -59,7 +59,7 @@ def test_wilson_mdagm(oracle, lat4):
 
 
 def test_wilson_cg_c1(oracle):
-    """Wilson 8^4, eps 0.3, kappa 0.12, CG to 1e-10 (C1)."""
+    """Wilson 8^4, links exp(i 0.3 H), kappa 0.12, CG to 1e-10."""
     lat = C.Lattice(L8, 1, (0, 0, 0, 1))
     u = wilson_gauge(L8, 0.3, 2024)
     lat.import_gauge(u)
@@ -235,7 +235,7 @@ SLOPPY_TOL = {C.PREC_MIXED_SINGLE: 1e-5, C.PREC_MIXED_HALF: 5e-3, C.PREC_MIXED_B
 
 
 @pytest.mark.parametrize("prec", [C.PREC_MIXED_SINGLE, C.PREC_MIXED_HALF, C.PREC_MIXED_BF16])
-@pytest.mark.parametrize("kind", ["wilson", "dwf", "mobius"])
+@pytest.mark.parametrize("kind", ["wilson", "dwf", "mobius", "clover"])
 def test_sloppy_operator(oracle, prec, kind):
     """The reduced-precision MdagM the mixed CG iterates with, against the FP64
     oracle; the tolerance is the storage rounding (FP32 ~1e-7, FP16 spinors
@@ -252,11 +252,198 @@ def test_sloppy_operator(oracle, prec, kind):
     elif kind == "dwf":
         op = C.make_op(C.OP_DWF, mass=0.01, m5=1.8)
         ref, dref = oracle.dwf_mdagm(u, inp, 0.01, oracle.dwf_struct(L, ls, 1.8))
-    else:
+    elif kind == "mobius":
         op = C.make_op(C.OP_MOBIUS, mass=0.01, m5=1.8, b=1.5, c=0.5)
         ref, dref = oracle.mobius_mdagm(u, inp, 0.01, oracle.mobius_struct(L, ls, 1.8, 1.5, 0.5))
+    else:
+        kappa, csw = 0.13, 1.0
+        op = C.make_op(C.OP_CLOVER, mass=1.0 / (2 * kappa) - 4.0, csw=csw)
+        lat.clover_build([kappa], csw)
+        A_oo = oracle.clover_mat_chkb(u, 1, 0, kappa, csw, L)
+        A_ee_inv = oracle.clover_mat_chkb(u, 0, 1, kappa, csw, L)
+        ref, dref = oracle.clover_mdagm(u, A_oo, A_ee_inv, inp, kappa, L)
     got, d = lat.mdagm_prec(op, inp, prec)
     print("sloppy", kind, prec, rel(got, ref), abs(d[0] - dref) / dref, flush=True)
     assert rel(got, ref) < SLOPPY_TOL[prec]
     assert abs(d[0] - dref) / dref < SLOPPY_TOL[prec]
     lat.close()
+
+
+# ---------------------------------------------------------------------------
+# clover (DiracOpClover, d_op_clover.C:107-352)
+
+CLOVER_KAPPAS = [0.1360, 0.1375]
+CLOVER_CSW = 1.0
+
+
+@pytest.fixture(scope="module")
+def clov4():
+    L = [4, 4, 4, 8]
+    lat = C.Lattice(L, 1, (0, 0, 0, 1))
+    u = wilson_gauge(L, 0.15, 41)
+    lat.import_gauge(u)
+    yield lat, u, L
+    lat.close()
+
+
+def clover_op(kappa, csw=CLOVER_CSW):
+    return C.make_op(C.OP_CLOVER, mass=1.0 / (2 * kappa) - 4.0, csw=csw)
+
+
+def test_clover_build_matches_oracle(oracle, clov4):
+    """A_oo and A_ee^-1 built on the device (two kappas at once) against
+    CloverMatChkb of the oracle; the inverse differs only by the rounding of
+    the factorisation."""
+    lat, u, L = clov4
+    lat.clover_build(CLOVER_KAPPAS, CLOVER_CSW)
+    for k, kappa in enumerate(CLOVER_KAPPAS):
+        for cb in (0, 1):
+            ref = oracle.clover_mat_chkb(u, cb, 1 if cb == 0 else 0, kappa, CLOVER_CSW, L)
+            got = lat.clover_export(k, cb)
+            print("clover build", k, cb, rel(got, ref), flush=True)
+            assert rel(got, ref) < 1e-13
+
+
+def test_clover_matpc_and_mdagm(oracle, clov4):
+    lat, u, L = clov4
+    kappa = 0.1365
+    lat.clover_build([kappa], CLOVER_CSW)
+    op = clover_op(kappa)
+    A_oo = oracle.clover_mat_chkb(u, 1, 0, kappa, CLOVER_CSW, L)
+    A_ee_inv = oracle.clover_mat_chkb(u, 0, 1, kappa, CLOVER_CSW, L)
+    inp = F.gaussian_fermion(lat.vcb, 42)
+    for dag in (0, 1):
+        ref = oracle.clover_matpc(u, A_oo, A_ee_inv, inp, kappa, dag, L)
+        assert rel(lat.matpc(op, inp, dag), ref) < 1e-13
+    ref, dp_ref = oracle.clover_mdagm(u, A_oo, A_ee_inv, inp, kappa, L)
+    got, dp = lat.mdagm(op, inp)
+    assert rel(got, ref) < 1e-13
+    assert abs(dp[0] - dp_ref) / dp_ref < 1e-13
+
+
+def test_clover_cg(oracle, clov4):
+    lat, u, L = clov4
+    kappa = 0.1365
+    lat.clover_build([kappa], CLOVER_CSW)
+    op = clover_op(kappa)
+    A_oo = oracle.clover_mat_chkb(u, 1, 0, kappa, CLOVER_CSW, L)
+    A_ee_inv = oracle.clover_mat_chkb(u, 0, 1, kappa, CLOVER_CSW, L)
+    src = F.gaussian_fermion(lat.vcb, 43)
+    ref, it_ref, _ = oracle.clover_invcg(u, A_oo, A_ee_inv, src, kappa, 1e-10, 5000, L)
+    got, res = lat.invcg(op, src, C.make_cg_arg(1e-10, 5000))
+    print("clover cg", it_ref, res[0].iterations, res[0].true_res, rel(got, ref), flush=True)
+    assert abs(res[0].iterations - it_ref) <= 1
+    assert res[0].true_res < 1e-10
+    assert rel(got, ref) < 1e-10
+    for prec in (C.PREC_MIXED_SINGLE, C.PREC_MIXED_HALF):
+        got2, res2 = lat.invcg(op, src, C.make_cg_arg(1e-10, 5000, precision=prec))
+        print("clover mixed", prec, res2[0].iterations, res2[0].true_res, rel(got2, ref), flush=True)
+        assert res2[0].true_res <= 1e-10
+        assert rel(got2, ref) < 1e-8
+
+
+def test_clover_multi_kappa_batch(clov4):
+    """Sources for two kappas in one batched solve (rhs r uses kappa r // 2)
+    give exactly the single-kappa solves."""
+    lat, u, L = clov4
+    srcs = [F.gaussian_fermion(lat.vcb, 50 + k) for k in range(4)]
+    arg = C.make_cg_arg(1e-10, 5000)
+    lat.clover_build(CLOVER_KAPPAS, CLOVER_CSW)
+    batched, res = lat.invcg(clover_op(CLOVER_KAPPAS[0]), np.concatenate(srcs), arg, nrhs=4)
+    n = 24 * lat.vcb
+    for r in range(4):
+        kappa = CLOVER_KAPPAS[r // 2]
+        lat.clover_build([kappa], CLOVER_CSW)
+        single, r1 = lat.invcg(clover_op(kappa), srcs[r], arg)
+        assert res[r].iterations == r1[0].iterations
+        assert np.array_equal(batched[r * n:(r + 1) * n], single)
+
+
+def test_clover_rejects_mismatched_kappa(clov4):
+    lat, u, L = clov4
+    lat.clover_build([0.1365], CLOVER_CSW)
+    with pytest.raises(C.CpsHipError):
+        lat.mdagm(clover_op(0.1370), F.gaussian_fermion(lat.vcb, 1))
+
+
+# ---------------------------------------------------------------------------
+# hipGraph replay of the CG iteration
+
+
+@pytest.mark.parametrize("prec", [C.PREC_DOUBLE, C.PREC_MIXED_HALF])
+@pytest.mark.parametrize("kind", ["dwf", "mobius", "clover"])
+def test_graph_replay_matches_eager(prec, kind):
+    """The captured iteration, replayed many times with host polls (and, for
+    mixed precision, eager reliable updates) between replays, gives exactly
+    the eager solve."""
+    L = [4, 4, 4, 8]
+    ls = 1 if kind == "clover" else 8
+    lat = C.Lattice(L, ls, (0, 0, 0, 1))
+    lat.import_gauge(wilson_gauge(L, 0.2, 120))
+    if kind == "dwf":
+        op = C.make_op(C.OP_DWF, mass=0.02, m5=1.8)
+    elif kind == "mobius":
+        op = C.make_op(C.OP_MOBIUS, mass=0.02, m5=1.8, b=1.5, c=0.5)
+    else:
+        lat.clover_build([0.13], 1.0)
+        op = C.make_op(C.OP_CLOVER, mass=1.0 / 0.26 - 4.0, csw=1.0)
+    src = np.concatenate([F.gaussian_fermion(lat.vcb * ls, 121 + k) for k in range(2)])
+    out = {}
+    for g in (0, 1):
+        arg = C.make_cg_arg(1e-10, 3000, precision=prec, reliable_every=40, check_every=4, use_graph=g)
+        out[g] = lat.invcg(op, src, arg, nrhs=2)
+    assert [r.iterations for r in out[0][1]] == [r.iterations for r in out[1][1]]
+    assert np.array_equal(out[0][0], out[1][0])
+    assert max(r.true_res for r in out[1][1]) <= 1e-10
+    lat.close()
+
+
+# ---------------------------------------------------------------------------
+# resident solve service (continuous batching)
+
+
+def test_service_results_same_alone_and_in_mix(oracle):
+    """Requests on two resident lattices with several operators and masses,
+    nrhs up to 5 with 4 slots per engine (slots are refilled mid-request):
+    every solution is bit-identical whether the request ran alone or in the
+    mix, meets the residual, and matches the FP64 oracle CG."""
+    LW, LD, ls = [8, 8, 8, 8], [4, 4, 4, 8], 8
+    uw = wilson_gauge(LW, 0.2, 140)
+    ud = wilson_gauge(LD, 0.2, 141)
+    svc = C.Service(8 << 30, max_slots=4)
+    gw = svc.add_gauge(LW, 1, uw)
+    gd = svc.add_gauge(LD, ls, ud)
+    vw, vd = 24 * int(np.prod(LW)) // 2, 24 * int(np.prod(LD)) // 2 * ls
+    reqs = []
+    for k, (g, mass, nrhs) in enumerate([(gw, 0.1, 1), (gd, 0.02, 5), (gw, 0.2, 3), (gd, 0.05, 2),
+                                          (gd, 0.02, 2), (gw, 0.1, 4)]):
+        if g == gw:
+            op = C.make_op(C.OP_WILSON, mass=mass)
+            src = np.concatenate([F.gaussian_fermion(vw // 24, 150 + 10 * k + r) for r in range(nrhs)])
+        else:
+            op = C.make_op(C.OP_DWF, mass=mass, m5=1.8)
+            src = np.concatenate([F.gaussian_fermion(vd // 24, 150 + 10 * k + r) for r in range(nrhs)])
+        reqs.append((g, op, src, nrhs, mass))
+    alone = []
+    for g, op, src, nrhs, _ in reqs:
+        rid = svc.submit(g, op, src, nrhs)
+        alone.append(svc.wait(rid))
+    ids = [svc.submit(g, op, src, nrhs) for g, op, src, nrhs, _ in reversed(reqs)]
+    mix = [svc.wait(rid) for rid in ids][::-1]
+    st = svc.stats()
+    print("service stats", st, flush=True)
+    for (g, op, src, nrhs, mass), (sa, ra, la), (sm, rm, lm) in zip(reqs, alone, mix):
+        assert np.array_equal(sa, sm)
+        assert [r.iterations for r in ra] == [r.iterations for r in rm]
+        assert max(r.true_res for r in rm) <= 1e-10
+        n = len(src) // nrhs
+        for r in range(nrhs):
+            if g == gw:
+                ref, _, _ = oracle.wilson_invcg(uw, src[r * n:(r + 1) * n], mass, 1e-10, 5000, LW)
+            else:
+                ref, _, _ = oracle.dwf_invcg(ud, src[r * n:(r + 1) * n], mass, 1e-10, 5000,
+                                             oracle.dwf_struct(LD, ls, 1.8))
+            assert rel(sm[r * n:(r + 1) * n], ref) < 1e-9
+    assert st["requests_done"] == 2 * len(reqs)
+    assert st["last_error"] == 0
+    svc.close()
