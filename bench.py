@@ -121,15 +121,102 @@ def cpu_baseline(cfg, u_w, src, gpu_iters):
                       f"{el:.1f} s; per-iteration time x {gpu_iters} iterations (the GPU FP64 count)"}
 
 
+SERVE_LATTICES = [([8, 8, 8, 8], 8), ([16, 16, 16, 32], 16), ([24, 24, 24, 48], 16)]
+SERVE_MASSES = [0.005, 0.01, 0.02, 0.05]
+
+
+def serve_stream(a):
+    """Solve-serving stream: requests with Poisson arrivals, each a Wilson or
+    DWF solve (Ls of the lattice's gauge entry) on one of three resident
+    lattices, a mass from SERVE_MASSES and 1-12 Gaussian sources, solved to
+    1e-10 by the resident service (continuous batching over CG slots).  The
+    latency of a request runs from its arrival (submit) to its last solution
+    being harvested."""
+    import threading
+    rng = np.random.default_rng(505)
+    svc = C.Service(200 << 30, max_slots=12)
+    gids = []
+    for L, ls in (SERVE_LATTICES if a.full else SERVE_LATTICES[:2]):
+        u = F.gauge_field(L, 0.2, 500 + len(gids))
+        gids.append(svc.add_gauge(L, ls, LY.gauge_canon_to_wilson(LY.bond_cond(u, L, [0, 0, 0, 1]), L)))
+    print("# service: gauges resident", file=sys.stderr, flush=True)
+    # request mix: the 24^3x48 lattice only with --full (its FP64 solves take
+    # minutes), so that the default stream finishes within the bench budget
+    probs = [0.5, 0.4, 0.1] if a.full else [0.6, 0.4, 0.0]
+    plan = []
+    t = 0.0
+    for k in range(a.requests):
+        t += rng.exponential(1.0 / a.rate)
+        li = int(rng.choice(3, p=probs))
+        kind = C.OP_WILSON if rng.random() < 0.5 else C.OP_DWF
+        mass = float(rng.choice(SERVE_MASSES))
+        nrhs = int(rng.integers(1, 13))
+        plan.append((t, li, kind, mass, nrhs))
+    lat_s, rhs_total, results = [], 0, []
+    lock = threading.Lock()
+    waiters = []
+
+    def waiter(rid, meta):
+        sol, res, lat = svc.wait(rid)
+        with lock:
+            lat_s.append(lat)
+            results.append((meta, max(r.true_res for r in res), [r.iterations for r in res]))
+
+    t0 = time.perf_counter()
+    for (ta, li, kind, mass, nrhs) in plan:
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        L, ls = SERVE_LATTICES[li]
+        vcb = int(np.prod(L)) // 2
+        per = vcb * (1 if kind == C.OP_WILSON else ls)
+        src = np.concatenate([F.gaussian_fermion(per, 7000 + 13 * len(waiters) + r) for r in range(nrhs)])
+        op = C.make_op(kind, mass=mass, m5=1.8)
+        rid = svc.submit(gids[li], op, src, nrhs)
+        rhs_total += nrhs
+        th = threading.Thread(target=waiter, args=(rid, (li, kind, mass, nrhs)))
+        th.start()
+        waiters.append(th)
+        if len(waiters) % 8 == 0:
+            print(f"# service: {len(waiters)} requests submitted, {len(lat_s)} done", file=sys.stderr, flush=True)
+    for th in waiters:
+        th.join()
+    wall = time.perf_counter() - t0
+    st = svc.stats()
+    svc.close()
+    lat = np.array(lat_s)
+    worst = max(r[1] for r in results)
+    line = {
+        "metric": "solve-serving stream: p50/p99 latency and solves/s (resident gauge, continuous batching)",
+        "value": len(lat) / wall, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "requests": len(lat), "rhs": rhs_total, "wall_s": wall, "rate_per_s": a.rate,
+        "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+        "rhs_per_s": rhs_total / wall, "true_res_max": worst, "dtype": "f64",
+        "data": "synthetic (seeded SU(3) links exp(i 0.2 H), Gaussian sources)",
+        "config": {"workload": "serve: Wilson/DWF on 8^4 (Ls 8), 16^3x32 (Ls 16)"
+                               + (", 24^3x48 (Ls 16)" if a.full else "") + "; masses "
+                               + ",".join(str(m) for m in SERVE_MASSES) + "; 1-12 rhs; stop_rsd 1e-10"},
+        "service": st,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c5"])
+    ap.add_argument("--requests", type=int, default=48, help="c5: number of requests in the stream")
+    ap.add_argument("--rate", type=float, default=1.0, help="c5: Poisson arrival rate (requests/s)")
+    ap.add_argument("--full", action="store_true",
+                    help="c5: include the 24^3x48 lattice in the mix (slow in FP64; off by default to keep the "
+                         "default run within minutes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
     ap.add_argument("--hop-reps", type=int, default=50)
+    ap.add_argument("--max-iter", type=int, default=20000,
+                    help="CG iteration cap (a small cap gives a bounded run for kernel traces)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the hopping kernels (mixed and FP64) and the reference copy kernel, "
                          "for hardware-counter runs")
@@ -138,6 +225,9 @@ def main():
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {a.gpus}",
               file=sys.stderr)
         sys.exit(2)
+    if a.config == "c5":
+        serve_stream(a)
+        return
     cfg = CONFIGS[a.config]
     print(f"# {cfg['name']}: building inputs", file=sys.stderr, flush=True)
     u_w, src = make_inputs(cfg)
@@ -159,7 +249,7 @@ def main():
         lat.close()
         return
     src_dev = C.DeviceBuffer.from_array(src)
-    arg_main = C.make_cg_arg(1e-10, 20000, precision=prec_main, reliable_every=100)
+    arg_main = C.make_cg_arg(1e-10, a.max_iter, precision=prec_main, reliable_every=100)
     t_main, res_main = time_solves(lat, op, src_dev, arg_main, nrhs, a.warmup, a.steps)
     ms = 1e3 * float(np.mean(t_main))
     s_per_rhs = float(np.mean(t_main)) / nrhs
@@ -168,7 +258,7 @@ def main():
     if mixed is not None and not a.no_fp64:
         # a few-iteration FP64 solve first, so that kernel autotuning is not timed
         time_solves(lat, op, src_dev, C.make_cg_arg(1e-10, 3, precision=C.PREC_DOUBLE), nrhs, 1, 0)
-        arg64 = C.make_cg_arg(1e-10, 20000, precision=C.PREC_DOUBLE)
+        arg64 = C.make_cg_arg(1e-10, a.max_iter, precision=C.PREC_DOUBLE)
         t64, res64 = time_solves(lat, op, src_dev, arg64, nrhs, 0, 1)
         fp64 = {"s_per_rhs": float(np.mean(t64)) / nrhs, "iterations": [r.iterations for r in res64],
                 "true_res_max": max(r.true_res for r in res64)}

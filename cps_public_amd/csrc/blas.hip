@@ -41,7 +41,7 @@ __device__ __forceinline__ void block_partial_d(double v, double* partial, int64
   const int64_t slot = (int64_t)rhs * gridDim.x + blockIdx.x;
 
 template <typename S>
-__global__ void k_norm(FieldRef x, int64_t per_rhs, double* partial) {
+__global__ void __launch_bounds__(256) k_norm(FieldRef x, int64_t per_rhs, double* partial) {
   ELEM_SETUP
   using R = typename SpinTraits<S>::R;
   cplx<R> v[12];
@@ -53,7 +53,7 @@ __global__ void k_norm(FieldRef x, int64_t per_rhs, double* partial) {
 }
 
 template <typename S>
-__global__ void k_redot(FieldRef x, FieldRef y, int64_t per_rhs, double* partial) {
+__global__ void __launch_bounds__(256) k_redot(FieldRef x, FieldRef y, int64_t per_rhs, double* partial) {
   ELEM_SETUP
   using R = typename SpinTraits<S>::R;
   cplx<R> a[12], b[12];
@@ -66,7 +66,7 @@ __global__ void k_redot(FieldRef x, FieldRef y, int64_t per_rhs, double* partial
 }
 
 // r = b - a (FP64), optional |r|^2
-__global__ void k_sub_norm_d(FieldRef b, FieldRef a, FieldRef r, int64_t per_rhs, double* partial) {
+__global__ void __launch_bounds__(256) k_sub_norm_d(FieldRef b, FieldRef a, FieldRef r, int64_t per_rhs, double* partial) {
   ELEM_SETUP
   cplx<double> vb[12], va[12];
   spin_load<SpinD, double>(b, e, vb);
@@ -82,32 +82,39 @@ __global__ void k_sub_norm_d(FieldRef b, FieldRef a, FieldRef r, int64_t per_rhs
 }
 
 // x += a p ; r -= a Ap ; |r|^2   (inv_cg.C:340-349), skipped for inactive rhs
-__global__ void k_cg_xr_d(FieldRef x, FieldRef p, FieldRef r, FieldRef ap, const double* alpha,
-                          const int* active, int64_t per_rhs, double* partial) {
+__global__ void __launch_bounds__(256) k_cg_xr_d(FieldRef x, FieldRef p, FieldRef r, FieldRef ap,
+                                                const double* alpha, const int* active, int64_t per_rhs,
+                                                double* partial) {
   ELEM_SETUP
   double n = 0;
   if (active[rhs]) {
     const double a = alpha[rhs];
-    cplx<double> vx[12], vp[12], vr[12], va[12];
-    spin_load<SpinD, double>(x, e, vx);
-    spin_load<SpinD, double>(p, e, vp);
-    spin_load<SpinD, double>(r, e, vr);
-    spin_load<SpinD, double>(ap, e, va);
+    {   // x += a p
+      cplx<double> vx[12], vp[12];
+      spin_load<SpinD, double>(x, e, vx);
+      spin_load<SpinD, double>(p, e, vp);
 #pragma unroll
-    for (int c = 0; c < 12; c++) {
-      vx[c] = cmk<double>(a * vp[c].re + vx[c].re, a * vp[c].im + vx[c].im);
-      vr[c] = cmk<double>(-a * va[c].re + vr[c].re, -a * va[c].im + vr[c].im);
-      n += vr[c].re * vr[c].re + vr[c].im * vr[c].im;
+      for (int c = 0; c < 12; c++) vx[c] = cmk<double>(a * vp[c].re + vx[c].re, a * vp[c].im + vx[c].im);
+      spin_store<SpinD, double>(x, e, vx);
     }
-    spin_store<SpinD, double>(x, e, vx);
-    spin_store<SpinD, double>(r, e, vr);
+    {   // r -= a Ap, |r|^2
+      cplx<double> vr[12], va[12];
+      spin_load<SpinD, double>(r, e, vr);
+      spin_load<SpinD, double>(ap, e, va);
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+        vr[c] = cmk<double>(-a * va[c].re + vr[c].re, -a * va[c].im + vr[c].im);
+        n += vr[c].re * vr[c].re + vr[c].im * vr[c].im;
+      }
+      spin_store<SpinD, double>(r, e, vr);
+    }
   }
   block_partial_d(n, partial, slot);
 }
 
 // p = b p + r for active rhs (inv_cg.C:360)
 template <typename S>
-__global__ void k_xpay(FieldRef p, FieldRef r, const double* beta, const int* active, int64_t per_rhs) {
+__global__ void __launch_bounds__(256) k_xpay(FieldRef p, FieldRef r, const double* beta, const int* active, int64_t per_rhs) {
   const int rhs = blockIdx.y;
   if (!active[rhs]) return;
   const int64_t e = (int64_t)rhs * per_rhs + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,7 +130,7 @@ __global__ void k_xpay(FieldRef p, FieldRef r, const double* beta, const int* ac
 
 // y = a*x + y (per-rhs a), any precision pair handled in compute precision of Y
 template <typename SX, typename SY>
-__global__ void k_axpy(FieldRef x, FieldRef y, const double* a, double a_const, int64_t per_rhs) {
+__global__ void __launch_bounds__(256) k_axpy(FieldRef x, FieldRef y, const double* a, double a_const, int64_t per_rhs) {
   const int rhs = blockIdx.y;
   const int64_t e = (int64_t)rhs * per_rhs + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   using RX = typename SpinTraits<SX>::R;
@@ -141,7 +148,7 @@ __global__ void k_axpy(FieldRef x, FieldRef y, const double* a, double a_const, 
 
 // xs += a p ; rs -= a ap ; |rs|^2 (sloppy CG step, any storage combination)
 template <typename SX, typename SP, typename SR, typename SA>
-__global__ void k_cg_xr_t(FieldRef x, FieldRef p, FieldRef r, FieldRef ap, const double* alpha,
+__global__ void __launch_bounds__(256) k_cg_xr_t(FieldRef x, FieldRef p, FieldRef r, FieldRef ap, const double* alpha,
                           const int* active, int64_t per_rhs, double* partial) {
   ELEM_SETUP
   double n = 0;
@@ -170,7 +177,7 @@ __global__ void k_cg_xr_t(FieldRef x, FieldRef p, FieldRef r, FieldRef ap, const
 
 // p = b p + r with p and r of different storage
 template <typename SP, typename SR>
-__global__ void k_xpay_t(FieldRef p, FieldRef r, const double* beta, const int* active, int64_t per_rhs) {
+__global__ void __launch_bounds__(256) k_xpay_t(FieldRef p, FieldRef r, const double* beta, const int* active, int64_t per_rhs) {
   const int rhs = blockIdx.y;
   if (!active[rhs]) return;
   const int64_t e = (int64_t)rhs * per_rhs + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,7 +195,7 @@ __global__ void k_xpay_t(FieldRef p, FieldRef r, const double* beta, const int* 
 
 // fill a field with zeros (used instead of memset so it can sit in captured graphs)
 template <typename S>
-__global__ void k_zero(FieldRef x, int64_t per_rhs) {
+__global__ void __launch_bounds__(256) k_zero(FieldRef x, int64_t per_rhs) {
   const int64_t e = (int64_t)blockIdx.y * per_rhs + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   using R = typename SpinTraits<S>::R;
   cplx<R> v[12];

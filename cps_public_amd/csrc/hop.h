@@ -46,6 +46,10 @@ struct HopParams {
   // CLOV_X out = xcoef*(A X) + kscale[set]*scale*hop, with A the clover set of
   // kappa index set = rhs / rhs_per_set at the output checkerboard and kscale
   // a device array (required for CLOV_X)
+  // CLOV_S5X epilogue (Wilson kind with nb = Ls, the Moebius layout):
+  // out = xcoef*X + x5coef*(P X) + scale*hop, P the fifth-dimension shift of
+  // the hop's dag with -x5mass at the walls
+  double x5coef, x5mass;
   const uint4* nbr;         // neighbour table (k_build_nbr)
   const uint32_t* nbr_neg;
   int clov;
@@ -54,7 +58,7 @@ struct HopParams {
   const double* kscale;
 };
 
-enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2 };
+enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2, CLOV_S5X = 3 };
 
 hipError_t launch_hop(const HopParams& P, hipStream_t st);
 hipError_t build_nbr(const Geom& g, uint4* nbr, uint32_t* nbr_neg, hipStream_t st);

@@ -1,15 +1,19 @@
-// Launchers of the Wilson hopping kernel with a fused clover epilogue
-// (out = A (scale hop) or out = xcoef A X + scale hop), the two halves of the
-// clover preconditioned operator A_oo - kappa^2 D A_ee^-1 D
-// (DiracOpClover::MatPcDagOrNot, d_op_clover.C:194-248).
+// Launchers of the Wilson hopping kernel with a fused epilogue: the clover
+// multiply (out = A (scale hop) or out = xcoef A X + scale hop), the two
+// halves of the clover preconditioned operator A_oo - kappa^2 D A_ee^-1 D
+// (DiracOpClover::MatPcDagOrNot, d_op_clover.C:194-248); or the Moebius
+// fifth-dimension term (out = X + r P X + scale hop, the last step of
+// zmobius_m_orig, zmobius_m-orig.h:3-122).
 #include "hop_kernel.h"
 
 namespace cpship {
 
 template <typename SIN, typename SOUT, typename LK, typename R>
 static hipError_t launch_c(const HopParams& P, hipStream_t st) {
-  if (P.kind != KIND_WILSON || P.rhs_per_set < 1 || !P.clover.data) return hipErrorInvalidValue;
+  if (P.kind != KIND_WILSON) return hipErrorInvalidValue;
+  if ((P.clov == CLOV_POST || P.clov == CLOV_X) && (P.rhs_per_set < 1 || !P.clover.data)) return hipErrorInvalidValue;
   if (P.clov == CLOV_X && !P.kscale) return hipErrorInvalidValue;
+  if (P.clov == CLOV_S5X && (!P.X.data || P.nb < 2)) return hipErrorInvalidValue;
   const int64_t nelem = (int64_t)P.nrhs * P.nsites_out * P.nb;
   const int bs = P.block;
   const int64_t nblk = (nelem + bs - 1) / bs;
@@ -22,6 +26,8 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, CLOV_POST) } else { CPSHIP_HOP_LAUNCH(0, CLOV_POST) }
   } else if (P.clov == CLOV_X) {
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, CLOV_X) } else { CPSHIP_HOP_LAUNCH(0, CLOV_X) }
+  } else if (P.clov == CLOV_S5X) {
+    if (P.dag) { CPSHIP_HOP_LAUNCH(1, CLOV_S5X) } else { CPSHIP_HOP_LAUNCH(0, CLOV_S5X) }
   } else {
     return hipErrorInvalidValue;
   }

@@ -608,7 +608,33 @@ __global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) 
 
     const int64_t eo = (int64_t)rhs * per_rhs + rem;
     cplx<R> res[12];
-    if constexpr (CLOV != CLOV_NONE) {
+    if constexpr (CLOV == CLOV_S5X) {
+      // j = s of the Moebius layout; the Ls slots of this site are adjacent
+      const int ls = P.nb;
+      const int64_t sbase = eo - j;
+      const int sm = (j == 0) ? ls - 1 : j - 1;
+      const int sp = (j == ls - 1) ? 0 : j + 1;
+      const R cm = (j == 0) ? R(-P.x5mass) : R(1);
+      const R cp = (j == ls - 1) ? R(-P.x5mass) : R(1);
+      // P of the hop's own dag (compile-time component offsets keep res[]
+      // in registers)
+      constexpr int c_from_m = DAG ? 6 : 0;
+      constexpr int c_from_p = DAG ? 0 : 6;
+      cplx<R> xv[12], hm[6], hp[6];
+      spin_load<SOUT, R>(P.X, eo, xv);
+      spin_load_half<SOUT, R>(P.X, sbase + sm, c_from_m, hm);
+      spin_load_half<SOUT, R>(P.X, sbase + sp, c_from_p, hp);
+      const R xc = R(P.xcoef), x5 = R(P.x5coef), sc = R(P.scale);
+#pragma unroll
+      for (int k = 0; k < 12; k++) res[k] = cmk<R>(xc * xv[k].re + sc * acc[k].re, xc * xv[k].im + sc * acc[k].im);
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        res[c_from_m + k].re += x5 * (cm * hm[k].re);
+        res[c_from_m + k].im += x5 * (cm * hm[k].im);
+        res[c_from_p + k].re += x5 * (cp * hp[k].re);
+        res[c_from_p + k].im += x5 * (cp * hp[k].im);
+      }
+    } else if constexpr (CLOV != CLOV_NONE) {
       const int set = rhs / P.rhs_per_set;
       // (no branch here: a conditional load splits the block and the stencil
       // above it then spills)

@@ -14,6 +14,7 @@
 //            zmobius kappa_dslash_5_plus) with -mass at the walls
 //   k_m5inv  in-place M5^-1; a block stages S sites x Ls slices in LDS, one
 //            thread per (site, component) runs the recurrence over s
+#include "hop.h"
 #include "solver.h"
 
 #include <cstring>
@@ -31,15 +32,14 @@ struct S5Params {
 
 // value of the CPS shift term (P+ from s-1, P- from s+1 for dag 0; exchanged
 // for dag 1) at slot s of a site whose Ls slots start at element `base`
-template <typename S, typename R>
-__device__ __forceinline__ void s5_term(const FieldRef& f, int64_t base, int s, int ls, int dag, R mass,
-                                        cplx<R> t[12]) {
+template <typename S, typename R, int DAG>
+__device__ __forceinline__ void s5_term(const FieldRef& f, int64_t base, int s, int ls, R mass, cplx<R> t[12]) {
   const int sm = (s == 0) ? ls - 1 : s - 1;
   const int sp = (s == ls - 1) ? 0 : s + 1;
   const R cm = (s == 0) ? -mass : R(1);
   const R cp = (s == ls - 1) ? -mass : R(1);
-  const int c_from_m = dag ? 6 : 0;
-  const int c_from_p = dag ? 0 : 6;
+  constexpr int c_from_m = DAG ? 6 : 0;
+  constexpr int c_from_p = DAG ? 0 : 6;
   cplx<R> hm[6], hp[6];
   spin_load_half<S, R>(f, base + sm, c_from_m, hm);
   spin_load_half<S, R>(f, base + sp, c_from_p, hp);
@@ -50,8 +50,8 @@ __device__ __forceinline__ void s5_term(const FieldRef& f, int64_t base, int s, 
   }
 }
 
-template <typename S>
-__global__ void k_s5(S5Params P) {
+template <typename S, int DAG>
+__global__ void __launch_bounds__(256) k_s5(S5Params P) {
   using R = typename SpinTraits<S>::R;
   const int rhs = blockIdx.y;
   const int64_t e = (int64_t)rhs * P.per_rhs + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -62,7 +62,7 @@ __global__ void k_s5(S5Params P) {
   {
     cplx<R> v[12], t[12];
     spin_load<S, R>(P.A, e, v);
-    s5_term<S, R>(P.A, base, s, P.ls, P.dag, mass, t);
+    s5_term<S, R, DAG>(P.A, base, s, P.ls, mass, t);
     const R a = R(P.a1), b = R(P.b1), c = R(P.c1);
 #pragma unroll
     for (int k = 0; k < 12; k++) res[k] = cmk<R>(a * (b * v[k].re + c * t[k].re), a * (b * v[k].im + c * t[k].im));
@@ -70,7 +70,7 @@ __global__ void k_s5(S5Params P) {
   if (P.B.data) {
     cplx<R> v[12], t[12];
     spin_load<S, R>(P.B, e, v);
-    s5_term<S, R>(P.B, base, s, P.ls, P.dag, mass, t);
+    s5_term<S, R, DAG>(P.B, base, s, P.ls, mass, t);
     const R a = R(P.a2), b = R(P.b2), c = R(P.c2);
 #pragma unroll
     for (int k = 0; k < 12; k++) {
@@ -107,11 +107,38 @@ __global__ void k_s5(S5Params P) {
 }
 
 struct M5Params {
-  FieldRef f;
+  FieldRef f;       // input
+  FieldRef out;     // output (may be f)
   int ls, dag;
   double K, mass;
   int64_t nsites;   // nrhs * Vcb
+  // fused fifth-dimension term t = b5 v + c5 P v (P of this dag, -mass at
+  // the walls): S5_NONE, S5_AFTER (t of the M5^-1 result is stored) or
+  // S5_BEFORE (M5^-1 is applied to t of the input)
+  int s5;
+  double b5, c5;
 };
+enum { S5_NONE = 0, S5_AFTER = 1, S5_BEFORE = 2 };
+
+// b5 v_s + c5 (P v)_s from the LDS copy of one site's Ls slots
+template <typename R, int DAG>
+__device__ __forceinline__ void s5_lds_t(const R* site_lds, int s, int ls, R mass, R b5, R c5, R t[24]) {
+  const int sm = (s == 0) ? ls - 1 : s - 1;
+  const int sp = (s == ls - 1) ? 0 : s + 1;
+  const R cm = (s == 0) ? -mass : R(1);
+  const R cp = (s == ls - 1) ? -mass : R(1);
+  // dag 0: reals 0-11 (spins 0,1) from s-1, 12-23 from s+1; dag 1 exchanged
+  constexpr int r_from_m = DAG ? 12 : 0, r_from_p = DAG ? 0 : 12;
+  const R* v = site_lds + s * 24;
+  const R* vm = site_lds + sm * 24;
+  const R* vp = site_lds + sp * 24;
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    t[r_from_m + k] = b5 * v[r_from_m + k] + c5 * (cm * vm[r_from_m + k]);
+    t[r_from_p + k] = b5 * v[r_from_p + k] + c5 * (cp * vp[r_from_p + k]);
+  }
+}
+
 
 // zmobius_m5inv_cmplx_dag0 / _dag1 with constant real K, one component
 template <typename R>
@@ -172,7 +199,7 @@ __device__ __forceinline__ void m5inv_component(R* v /* [s] stride 12 */, int st
 }
 
 // block: SITES sites x ls slots; LDS holds the block's fields in compute precision
-template <typename S>
+template <typename S, int DAG>
 __global__ void k_m5inv(M5Params P) {
   using R = typename SpinTraits<S>::R;
   extern __shared__ unsigned char smem_raw[];
@@ -181,40 +208,53 @@ __global__ void k_m5inv(M5Params P) {
   const int sites_per_block = blockDim.x / ls;
   const int64_t site0 = (int64_t)blockIdx.x * sites_per_block;
   // phase 1: element (site_local, s) -> LDS
-  {
-    const int sl = threadIdx.x / ls, s = threadIdx.x - (threadIdx.x / ls) * ls;
-    const int64_t site = site0 + sl;
-    if (site < P.nsites) {
-      cplx<R> v[12];
-      spin_load<S, R>(P.f, site * ls + s, v);
-      R* d = lds + ((int64_t)sl * ls + s) * 24;
+  const int sl_t = threadIdx.x / ls, s_t = threadIdx.x - (threadIdx.x / ls) * ls;
+  const int64_t site_t = site0 + sl_t;
+  const bool mine = sl_t < sites_per_block && site_t < P.nsites;
+  if (mine) {
+    cplx<R> v[12];
+    spin_load<S, R>(P.f, site_t * ls + s_t, v);
+    R* d = lds + ((int64_t)sl_t * ls + s_t) * 24;
 #pragma unroll
-      for (int c = 0; c < 12; c++) {
-        d[2 * c] = v[c].re;
-        d[2 * c + 1] = v[c].im;
-      }
+    for (int c = 0; c < 12; c++) {
+      d[2 * c] = v[c].re;
+      d[2 * c + 1] = v[c].im;
     }
   }
   __syncthreads();
+  if (P.s5 == S5_BEFORE) {
+    R t[24];
+    if (mine) s5_lds_t<R, DAG>(lds + (int64_t)sl_t * ls * 24, s_t, ls, R(P.mass), R(P.b5), R(P.c5), t);
+    __syncthreads();
+    if (mine) {
+      R* d = lds + ((int64_t)sl_t * ls + s_t) * 24;
+#pragma unroll
+      for (int k = 0; k < 24; k++) d[k] = t[k];
+    }
+    __syncthreads();
+  }
   // phase 2: (site_local, real component r in 0..23) runs the recurrence
   for (int w = threadIdx.x; w < sites_per_block * 24; w += blockDim.x) {
     const int sl = w / 24, r = w - sl * 24;
     if (site0 + sl < P.nsites) {
       const int lower = r >= 12;   // spins 2,3
-      m5inv_component<R>(lds + (int64_t)sl * ls * 24 + r, 24, ls, P.dag, lower, R(P.K), R(P.mass));
+      m5inv_component<R>(lds + (int64_t)sl * ls * 24 + r, 24, ls, DAG, lower, R(P.K), R(P.mass));
     }
   }
   __syncthreads();
-  {
-    const int sl = threadIdx.x / ls, s = threadIdx.x - (threadIdx.x / ls) * ls;
-    const int64_t site = site0 + sl;
-    if (site < P.nsites) {
-      cplx<R> v[12];
-      const R* d = lds + ((int64_t)sl * ls + s) * 24;
+  if (mine) {
+    cplx<R> v[12];
+    if (P.s5 == S5_AFTER) {
+      R t[24];
+      s5_lds_t<R, DAG>(lds + (int64_t)sl_t * ls * 24, s_t, ls, R(P.mass), R(P.b5), R(P.c5), t);
+#pragma unroll
+      for (int c = 0; c < 12; c++) v[c] = cmk<R>(t[2 * c], t[2 * c + 1]);
+    } else {
+      const R* d = lds + ((int64_t)sl_t * ls + s_t) * 24;
 #pragma unroll
       for (int c = 0; c < 12; c++) v[c] = cmk<R>(d[2 * c], d[2 * c + 1]);
-      spin_store<S, R>(P.f, site * ls + s, v);
     }
+    spin_store<S, R>(P.out, site_t * ls + s_t, v);
   }
 }
 
@@ -224,17 +264,26 @@ static int launch_s5(Lattice& L, int hprec, S5Params& P) {
   const int bs = pick_block(P.per_rhs);
   dim3 grid((unsigned)(P.per_rhs / bs), (unsigned)P.nrhs), block(bs);
   switch (store_of(hprec)) {
-    case STORE_D: hipLaunchKernelGGL(k_s5<SpinD>, grid, block, 0, L.stream, P); break;
-    case STORE_F: hipLaunchKernelGGL(k_s5<SpinF>, grid, block, 0, L.stream, P); break;
-    default: hipLaunchKernelGGL(k_s5<SpinH>, grid, block, 0, L.stream, P); break;
+#define CPSHIP_S5(S)                                                                   \
+  if (P.dag) hipLaunchKernelGGL((k_s5<S, 1>), grid, block, 0, L.stream, P);            \
+  else hipLaunchKernelGGL((k_s5<S, 0>), grid, block, 0, L.stream, P);
+    case STORE_D: CPSHIP_S5(SpinD) break;
+    case STORE_F: CPSHIP_S5(SpinF) break;
+    default: CPSHIP_S5(SpinH) break;
+#undef CPSHIP_S5
   }
   CPSHIP_CHECK(hipGetLastError());
   return 0;
 }
 
-static int launch_m5inv(Lattice& L, int hprec, const FieldRef& f, int dag, double K, double mass, int nrhs) {
+static int launch_m5inv(Lattice& L, int hprec, const FieldRef& f, int dag, double K, double mass, int nrhs,
+                        const FieldRef* out = nullptr, int s5 = S5_NONE, double b5 = 0.0, double c5 = 0.0) {
   M5Params P;
   P.f = f;
+  P.out = out ? *out : f;
+  P.s5 = s5;
+  P.b5 = b5;
+  P.c5 = c5;
   P.ls = L.ls;
   P.dag = dag;
   P.K = K;
@@ -247,9 +296,13 @@ static int launch_m5inv(Lattice& L, int hprec, const FieldRef& f, int dag, doubl
   const size_t lds = (size_t)sites_per_block * L.ls * 24 * rbytes;
   dim3 grid((unsigned)((P.nsites + sites_per_block - 1) / sites_per_block)), block(bs);
   switch (st) {
-    case STORE_D: hipLaunchKernelGGL(k_m5inv<SpinD>, grid, block, lds, L.stream, P); break;
-    case STORE_F: hipLaunchKernelGGL(k_m5inv<SpinF>, grid, block, lds, L.stream, P); break;
-    default: hipLaunchKernelGGL(k_m5inv<SpinH>, grid, block, lds, L.stream, P); break;
+#define CPSHIP_M5(S)                                                                   \
+  if (dag) hipLaunchKernelGGL((k_m5inv<S, 1>), grid, block, lds, L.stream, P);         \
+  else hipLaunchKernelGGL((k_m5inv<S, 0>), grid, block, lds, L.stream, P);
+    case STORE_D: CPSHIP_M5(SpinD) break;
+    case STORE_F: CPSHIP_M5(SpinF) break;
+    default: CPSHIP_M5(SpinH) break;
+#undef CPSHIP_M5
   }
   CPSHIP_CHECK(hipGetLastError());
   return 0;
@@ -261,9 +314,12 @@ static void mobius_kappas(const cps_hip_op* op, double* kb, double* kc) {
   *kc = 1.0 / (2 * (op->mobius_c * (4 - op->m5) - op->a5_inv));
 }
 
-// 4-D hop of every slice (Wilson kernel, nb = Ls), output parity `target`
+// 4-D hop of every slice (Wilson kernel, nb = Ls), output parity `target`;
+// with s5x the epilogue adds X + x5coef P X (P of direction dag, -mass walls)
+// and writes the per-block |out|^2 partials when partial is given
 static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag, double scale,
-                int nrhs) {
+                int nrhs, bool s5x = false, const FieldRef* X = nullptr, double x5coef = 0.0, double mass = 0.0,
+                double* partial = nullptr) {
   cps_hip_op w;
   std::memset(&w, 0, sizeof(w));
   w.kind = CPS_HIP_OP_MOBIUS;   // single-parity layout with nb = Ls
@@ -283,6 +339,14 @@ static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, 
   P.in = in;
   P.out = out;
   P.X = null_field();
+  P.partial = partial;
+  if (s5x) {
+    P.clov = CLOV_S5X;
+    P.X = *X;
+    P.xcoef = 1.0;
+    P.x5coef = x5coef;
+    P.x5mass = mass;
+  }
   const int lstore = link_storage_of(hprec);
   int rc = L.ensure_gauge(lstore);
   if (rc) return rc;
@@ -311,32 +375,22 @@ int mobius_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, c
   const double b = op->mobius_b, c = op->mobius_c;
   int rc;
   if (dag == 0) {
-    // t1 = b in + c P in ; out = kb * D4(cb=1) t1 ; M5^-1 out
+    // t1 = b in + c P in ; out = kb * D4(cb=1) t1
     S5Params P = s5_base(L, op, 0, nrhs);
     P.out = t1; P.A = in; P.a1 = 1; P.b1 = b; P.c1 = c;
     if ((rc = launch_s5(L, hprec, P))) return rc;
     if ((rc = hop4(L, hprec, out, t1, 0, 0, kb, nrhs))) return rc;
-    if ((rc = launch_m5inv(L, hprec, out, 0, ratio, op->mass, nrhs))) return rc;
-    // t1 = b out + c P out ; t2 = -kb * D4(cb=0) t1
-    P = s5_base(L, op, 0, nrhs);
-    P.out = t1; P.A = out; P.a1 = 1; P.b1 = b; P.c1 = c;
-    if ((rc = launch_s5(L, hprec, P))) return rc;
-    if ((rc = hop4(L, hprec, t2, t1, 1, 0, -kb, nrhs))) return rc;
-    // out = in + t2 + (kb/kc) P in
-    P = s5_base(L, op, 0, nrhs);
-    P.out = out; P.A = in; P.a1 = 1; P.b1 = 1; P.c1 = ratio; P.X = t2; P.xc = 1;
-    P.partial = partial;
-    return launch_s5(L, hprec, P);
+    // t1 = b M5^-1 out + c P M5^-1 out (one kernel)
+    if ((rc = launch_m5inv(L, hprec, out, 0, ratio, op->mass, nrhs, &t1, S5_AFTER, b, c))) return rc;
+    // out = in + (kb/kc) P in - kb * D4(cb=0) t1 (the hop's epilogue)
+    return hop4(L, hprec, out, t1, 1, 0, -kb, nrhs, true, &in, ratio, op->mass, partial);
   }
-  // dag 1: t1 = -kb * D4dag(cb=1) in ; t2 = b t1 + c Pdag t1 ; M5^-dag t2
+  // dag 1: t1 = -kb * D4dag(cb=1) in ; t2 = M5^-dag (b t1 + c Pdag t1)
   if ((rc = hop4(L, hprec, t1, in, 0, 1, -kb, nrhs))) return rc;
-  S5Params P = s5_base(L, op, 1, nrhs);
-  P.out = t2; P.A = t1; P.a1 = 1; P.b1 = b; P.c1 = c;
-  if ((rc = launch_s5(L, hprec, P))) return rc;
-  if ((rc = launch_m5inv(L, hprec, t2, 1, ratio, op->mass, nrhs))) return rc;
+  if ((rc = launch_m5inv(L, hprec, t1, 1, ratio, op->mass, nrhs, &t2, S5_BEFORE, b, c))) return rc;
   // t1 = kb * D4dag(cb=0) t2 ; out = (b t1 + c Pdag t1) + (in + ratio Pdag in)
   if ((rc = hop4(L, hprec, t1, t2, 1, 1, kb, nrhs))) return rc;
-  P = s5_base(L, op, 1, nrhs);
+  S5Params P = s5_base(L, op, 1, nrhs);
   P.out = out; P.A = t1; P.a1 = 1; P.b1 = b; P.c1 = c;
   P.B = in; P.a2 = 1; P.b2 = 1; P.c2 = ratio;
   P.partial = partial;
