@@ -31,6 +31,8 @@ hipError_t blas_cg_xr_d(const FieldRef& x, const FieldRef& p, const FieldRef& r,
                         int64_t per_rhs, int nrhs, int bs, double* partial, hipStream_t s);
 hipError_t blas_xpay(int st, const FieldRef& p, const FieldRef& r, const double* beta, const int* active,
                      int64_t per_rhs, int nrhs, int bs, hipStream_t s);
+hipError_t blas_redot_mixed(int sx, const FieldRef& x, int sy, const FieldRef& y, int64_t per_rhs, int nrhs, int bs,
+                            double* partial, hipStream_t s);
 hipError_t blas_axpy(int sx, const FieldRef& x, int sy, const FieldRef& y, const double* a, double a_const,
                      int64_t per_rhs, int nrhs, int bs, hipStream_t s);
 hipError_t reduce_rhs(const double* partial, int bpr, int nrhs, double* out, int mode, const CGScalars& cg,

@@ -65,6 +65,22 @@ __global__ void __launch_bounds__(256) k_redot(FieldRef x, FieldRef y, int64_t p
   block_partial_d(n, partial, slot);
 }
 
+// Re <x, y> with x and y in different storages
+template <typename SX, typename SY>
+__global__ void __launch_bounds__(256) k_redot_t(FieldRef x, FieldRef y, int64_t per_rhs, double* partial) {
+  ELEM_SETUP
+  using RX = typename SpinTraits<SX>::R;
+  using RY = typename SpinTraits<SY>::R;
+  cplx<RX> a[12];
+  cplx<RY> b[12];
+  spin_load<SX, RX>(x, e, a);
+  spin_load<SY, RY>(y, e, b);
+  double n = 0;
+#pragma unroll
+  for (int c = 0; c < 12; c++) n += double(a[c].re) * double(b[c].re) + double(a[c].im) * double(b[c].im);
+  block_partial_d(n, partial, slot);
+}
+
 // r = b - a (FP64), optional |r|^2
 __global__ void __launch_bounds__(256) k_sub_norm_d(FieldRef b, FieldRef a, FieldRef r, int64_t per_rhs, double* partial) {
   ELEM_SETUP
@@ -311,6 +327,16 @@ hipError_t blas_xpay(int st, const FieldRef& p, const FieldRef& r, const double*
   if (st == STORE_D) hipLaunchKernelGGL(k_xpay<SpinD>, egrid(per_rhs, bs, nrhs), dim3(bs), 0, s, p, r, beta, active, per_rhs);
   else if (st == STORE_F) hipLaunchKernelGGL(k_xpay<SpinF>, egrid(per_rhs, bs, nrhs), dim3(bs), 0, s, p, r, beta, active, per_rhs);
   else hipLaunchKernelGGL(k_xpay<SpinH>, egrid(per_rhs, bs, nrhs), dim3(bs), 0, s, p, r, beta, active, per_rhs);
+  return hipGetLastError();
+}
+
+hipError_t blas_redot_mixed(int sx, const FieldRef& x, int sy, const FieldRef& y, int64_t per_rhs, int nrhs, int bs,
+                            double* partial, hipStream_t s) {
+  dim3 g = egrid(per_rhs, bs, nrhs), b(bs);
+  if (sx == STORE_F && sy == STORE_H) hipLaunchKernelGGL((k_redot_t<SpinF, SpinH>), g, b, 0, s, x, y, per_rhs, partial);
+  else if (sx == STORE_F && sy == STORE_F) hipLaunchKernelGGL((k_redot_t<SpinF, SpinF>), g, b, 0, s, x, y, per_rhs, partial);
+  else if (sx == STORE_D && sy == STORE_D) hipLaunchKernelGGL((k_redot_t<SpinD, SpinD>), g, b, 0, s, x, y, per_rhs, partial);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

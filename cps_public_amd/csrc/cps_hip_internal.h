@@ -204,7 +204,10 @@ __device__ __forceinline__ void spin_store(const FieldRef& f, int64_t e, const c
     float m = 0.f;
 #pragma unroll
     for (int c = 0; c < 12; c++) m = fmaxf(m, fmaxf(fabsf(float(v[c].re)), fabsf(float(v[c].im))));
-    const float inv = m > 0.f ? 1.0f / m : 0.f;
+    // a site whose largest component is an FP32 subnormal (far from a point
+    // source the residual decays below 1e-38) is stored as zero: 1/m would
+    // overflow to inf and the stored halves would become inf/NaN
+    const float inv = m >= 1.17549435e-38f ? 1.0f / m : 0.f;
     uint4* p = reinterpret_cast<uint4*>(f.data) + e;
 #pragma unroll
     for (int k = 0; k < 3; k++) {

@@ -593,17 +593,20 @@ static void mixed_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int*
   precision_formats(prec, op, ls, nrhs, sp_store, hprec);
 }
 
-// After a reliable update (rr_true = FP64 |src - MdagM x|^2 per rhs):
+// After a reliable update (rr_true = FP64 |src - MdagM x|^2 per rhs, the
+// sloppy residual already replaced by the FP64 one, rp = <r, p> with the
+// direction p of the last step, whose own update p = r + beta p is still to
+// come at the start of the next iteration):
 //  - a rhs whose FP64 residual meets the stopping rule is finished;
-//  - a rhs that paused inside the last chunk (sloppy convergence or the
-//    delta trigger) skipped its direction update there, so it gets it now,
-//    p = r + beta p with beta = |r|^2 / |r_prev|^2 from the FP64 residual
-//    (mask = 1 marks those rhs for the xpay launched next);
-//  - every unfinished rhs continues from the FP64 residual norm.
-__global__ void k_mixed_after_update(CGScalars cg, const double* rr_true, int* mask, int nrhs) {
+//  - every other unfinished rhs continues from the FP64 residual: |r|^2 is
+//    the true one, beta = |r|^2 / |r_prev|^2, and p is re-orthogonalised
+//    against the new r, p -= (<r,p>/|r|^2) r (coef[k]; 0 for the others), as
+//    reliable-update CG does so that the sloppy drift of r does not break
+//    the CG relations of the next step.
+__global__ void k_mixed_after_update(CGScalars cg, const double* rr_true, const double* rp, double* coef, int nrhs) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nrhs) return;
-  mask[k] = 0;
+  coef[k] = 0.0;
   const int st = cg.status[k];
   if (st == CG_STATUS_D_ZERO || st == CG_STATUS_MAXITER) return;
   if (st == CG_STATUS_CONVERGED && !cg.active[k] && cg.iters[k] == 0 && rr_true[k] <= cg.stp[k]) return;
@@ -612,14 +615,12 @@ __global__ void k_mixed_after_update(CGScalars cg, const double* rr_true, int* m
   if (v <= cg.stp[k]) {
     cg.rr[k] = v;
     cg.active[k] = 0;
+    cg.beta[k] = 0.0;
     cg.status[k] = CG_STATUS_CONVERGED;
     return;
   }
-  if (!cg.active[k]) {
-    // paused during the chunk: finish its interrupted iteration
-    cg.beta[k] = v / cg.rr_prev[k];
-    mask[k] = 1;
-  }
+  cg.beta[k] = v / cg.rr_prev[k];
+  coef[k] = v > 0.0 ? -rp[k] / v : 0.0;
   cg.rr[k] = v;
   cg.active[k] = 1;
   cg.status[k] = CG_STATUS_RUNNING;
@@ -677,6 +678,15 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), STORE_F, rs.ref(), n, L.stream));
   CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), sps, ps.ref(), n, L.stream));
   CPSHIP_CHECK(blas_zero(STORE_F, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
+  {
+    // the first iteration's direction update p = r + 0 p
+    std::vector<double> zeros(nrhs, 0.0);
+    CPSHIP_CHECK(hipMemcpyAsync(cg.s.beta, zeros.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, L.stream));
+  }
+  double* rp = cg.s.alpha;       // <r, p> at a reliable update (alpha is rewritten by the next step)
+  DevBuf coef_buf;
+  if ((rc = coef_buf.ensure(nrhs * sizeof(double)))) return rc;
+  double* coef = (double*)coef_buf.p;
 
   const int chunk = arg->check_every > 0 ? arg->check_every : auto_check_every(o, nrhs);
   const int every = arg->reliable_every > 0 ? arg->reliable_every : 100;
@@ -696,15 +706,18 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   while (any(live) && done < max_itr) {
     prog.tick(L, done, live, cg.s.rr);
     const int todo = std::min(std::min(chunk, max_itr - done), every - since);
+    // one iteration: the previous step's direction update p = r + beta p
+    // first, then Ap, alpha, x and r, |r|^2 and beta -- so that a reliable
+    // update at a chunk boundary falls between r's update and p's
     rc = iter_graph.run(L, todo, [&]() -> int {
+      CPSHIP_CHECK(blas_xpay_mixed(sps, ps.ref(), STORE_F, rs.ref(), cg.s.beta, cg.s.active, o.per_rhs, nrhs, bsb,
+                                   L.stream));
       int e = apply_mdagm(L, op, hprec, aps.ref(), ps.ref(), t1s.ref(), t2s.ref(), part, nrhs);
       if (e) return e;
       CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nrhs, nullptr, RED_CG_ALPHA, cg.s, L.stream));
       CPSHIP_CHECK(blas_cg_xr_mixed(STORE_F, xs.ref(), sps, ps.ref(), STORE_F, rs.ref(), sps, aps.ref(), cg.s,
                                     o.per_rhs, nrhs, bsb, part, L.stream));
       CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_CG_BETA, cg.s, L.stream));
-      CPSHIP_CHECK(blas_xpay_mixed(sps, ps.ref(), STORE_F, rs.ref(), cg.s.beta, cg.s.active, o.per_rhs, nrhs, bsb,
-                                   L.stream));
       return 0;
     });
     if (rc) return rc;
@@ -735,12 +748,13 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
       CPSHIP_CHECK(blas_sub_norm_d(src.ref(), ap.ref(), r.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
       CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, rr_true, RED_PLAIN, cg.s, L.stream));
       CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), STORE_F, rs.ref(), n, L.stream));
-      hipLaunchKernelGGL(k_mixed_after_update, dim3((nrhs + 63) / 64), dim3(64), 0, L.stream, cg.s, rr_true, cg.mask,
+      CPSHIP_CHECK(blas_redot_mixed(STORE_F, rs.ref(), sps, ps.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, rp, RED_PLAIN, cg.s, L.stream));
+      hipLaunchKernelGGL(k_mixed_after_update, dim3((nrhs + 63) / 64), dim3(64), 0, L.stream, cg.s, rr_true, rp, coef,
                          nrhs);
       CPSHIP_CHECK(hipGetLastError());
-      // the direction update that the paused rhs skipped
-      CPSHIP_CHECK(blas_xpay_mixed(sps, ps.ref(), STORE_F, rs.ref(), cg.s.beta, cg.mask, o.per_rhs, nrhs, bsb,
-                                   L.stream));
+      // p -= (<r,p>/|r|^2) r for the continuing rhs (coef 0 for the others)
+      CPSHIP_CHECK(blas_axpy(STORE_F, rs.ref(), sps, ps.ref(), coef, 0.0, o.per_rhs, nrhs, bsb, L.stream));
       for (int k = 0; k < nrhs; k++)
         if (live[k]) n_updates[k]++;
       since = 0;

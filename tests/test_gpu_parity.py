@@ -447,3 +447,22 @@ def test_service_results_same_alone_and_in_mix(oracle):
     assert st["requests_done"] == 2 * len(reqs)
     assert st["last_error"] == 0
     svc.close()
+
+
+def test_half_storage_subnormal_site():
+    """A site whose values are FP32 subnormals (as far from a point source on
+    a large lattice) is stored as zero in FP16 storage, not as inf/NaN: the
+    sloppy operator stays finite and within its tolerance of FP64."""
+    L, ls = [4, 4, 4, 8], 8
+    lat = C.Lattice(L, ls, (0, 0, 0, 1))
+    lat.import_gauge(wilson_gauge(L, 0.2, 160))
+    op = C.make_op(C.OP_DWF, mass=0.02, m5=1.8)
+    inp = F.gaussian_fermion(lat.vcb * ls, 161)
+    inp[24 * 5:24 * 6] *= 1e-40          # one 5-D site of subnormal magnitude
+    inp[24 * 77:24 * 78] *= 3e-39
+    ref, dref = lat.mdagm(op, inp)
+    for prec in (C.PREC_MIXED_HALF, C.PREC_MIXED_BF16):
+        got, d = lat.mdagm_prec(op, inp, prec)
+        assert np.isfinite(got).all()
+        assert rel(got, ref) < SLOPPY_TOL[prec]
+    lat.close()
