@@ -15,6 +15,10 @@ Workloads (--config):
   c3            Moebius DWF 24^3x64, Ls=16, b=1.5, c=0.5, M5=1.8, mf=0.005,
                 12 point sources (spin x colour) in one batched solve, BF16
                 links, FP16 spinors.
+  c3h           c3 with FP16 links.
+  c4            clover 32^3x64, c_sw 1.0, links exp(i 0.15 H), 12 point sources
+                x 4 kappas (0.1360-0.1375) = 48 rhs in one batched solve, FP16
+                spinors and links.
   c1            Wilson 8^4, kappa 0.12, links exp(i 0.3 H), FP64 only.
 """
 import argparse
@@ -44,6 +48,14 @@ CONFIGS = {
     "c3": dict(name="mobius_dwf_24^3x64_ls16_b1.5_c0.5_mf0.005_12rhs", L=[24, 24, 24, 64], ls=16,
                kind=C.OP_MOBIUS, eps=0.2, mass=0.005, m5=1.8, b=1.5, c=0.5, nrhs=12, src="point",
                mixed=C.PREC_MIXED_BF16, seed=303),
+    # the same workload with FP16 instead of BF16 link storage (same bytes;
+    # BF16's 8-bit mantissa costs 2.5x the CG iterations at this mass)
+    "c3h": dict(name="mobius_dwf_24^3x64_ls16_b1.5_c0.5_mf0.005_12rhs_fp16links", L=[24, 24, 24, 64], ls=16,
+                kind=C.OP_MOBIUS, eps=0.2, mass=0.005, m5=1.8, b=1.5, c=0.5, nrhs=12, src="point",
+                mixed=C.PREC_MIXED_HALF, seed=303),
+    "c4": dict(name="clover_32^3x64_csw1.0_kappa0.1360-0.1375_48rhs", L=[32, 32, 32, 64], ls=1,
+               kind=C.OP_CLOVER, eps=0.15, kappas=[0.1360, 0.1365, 0.1370, 0.1375], csw=1.0, nrhs=48,
+               src="point", mixed=C.PREC_MIXED_HALF, seed=404),
 }
 
 
@@ -67,6 +79,9 @@ def make_inputs(cfg):
 
 
 def op_for(cfg):
+    if cfg["kind"] == C.OP_CLOVER:
+        # rhs r uses kappas[r // (nrhs / len(kappas))]; the op names the first
+        return C.make_op(C.OP_CLOVER, mass=1.0 / (2 * cfg["kappas"][0]) - 4.0, csw=cfg["csw"])
     return C.make_op(cfg["kind"], mass=cfg["mass"], m5=cfg.get("m5", 1.8), b=cfg.get("b", 1.0),
                      c=cfg.get("c", 0.0))
 
@@ -103,7 +118,12 @@ def cpu_baseline(cfg, u_w, src, gpu_iters):
     L, ls = cfg["L"], cfg["ls"]
     per = 24 * (int(np.prod(L)) // 2) * ls
     s0 = np.ascontiguousarray(src[:per])
-    n_it = 3 if ls > 1 else 50
+    n_it = 3 if ls > 1 or np.prod(L) > 65536 else 50
+    if cfg["kind"] == C.OP_CLOVER:
+        # the clover matrices are set-up work (CloverMatChkb), outside the clock
+        kappa = cfg["kappas"][0]
+        A_oo = O.clover_mat_chkb(u_w, 1, 0, kappa, cfg["csw"], L)
+        A_ee_inv = O.clover_mat_chkb(u_w, 0, 1, kappa, cfg["csw"], L)
     t0 = time.perf_counter()
     if cfg["kind"] == C.OP_DWF:
         d = O.dwf_struct(L, ls, cfg["m5"])
@@ -111,6 +131,8 @@ def cpu_baseline(cfg, u_w, src, gpu_iters):
     elif cfg["kind"] == C.OP_MOBIUS:
         m = O.mobius_struct(L, ls, cfg["m5"], cfg["b"], cfg["c"])
         _, it, _ = O.mobius_invcg(u_w, s0, cfg["mass"], 1e-30, n_it + 1, m)
+    elif cfg["kind"] == C.OP_CLOVER:
+        _, it, _ = O.clover_invcg(u_w, A_oo, A_ee_inv, s0, kappa, 1e-30, n_it + 1, L)
     else:
         _, it, _ = O.wilson_invcg(u_w, s0, cfg["mass"], 1e-30, n_it + 1, L)
     el = time.perf_counter() - t0
@@ -234,6 +256,8 @@ def main():
     print("# inputs ready", file=sys.stderr, flush=True)
     lat = C.Lattice(cfg["L"], cfg["ls"], (0, 0, 0, 1))
     lat.import_gauge(u_w)
+    if cfg["kind"] == C.OP_CLOVER:
+        lat.clover_build(cfg["kappas"], cfg["csw"])
     op = op_for(cfg)
     nrhs = cfg["nrhs"]
 

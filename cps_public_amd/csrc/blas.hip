@@ -336,6 +336,7 @@ hipError_t blas_redot_mixed(int sx, const FieldRef& x, int sy, const FieldRef& y
   if (sx == STORE_F && sy == STORE_H) hipLaunchKernelGGL((k_redot_t<SpinF, SpinH>), g, b, 0, s, x, y, per_rhs, partial);
   else if (sx == STORE_F && sy == STORE_F) hipLaunchKernelGGL((k_redot_t<SpinF, SpinF>), g, b, 0, s, x, y, per_rhs, partial);
   else if (sx == STORE_D && sy == STORE_D) hipLaunchKernelGGL((k_redot_t<SpinD, SpinD>), g, b, 0, s, x, y, per_rhs, partial);
+  else if (sx == STORE_H && sy == STORE_H) hipLaunchKernelGGL((k_redot_t<SpinH, SpinH>), g, b, 0, s, x, y, per_rhs, partial);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -374,6 +375,7 @@ hipError_t blas_cg_xr_mixed(int sx, const FieldRef& x, int sp, const FieldRef& p
   CPSHIP_XR(STORE_F, SpinF, STORE_H, SpinH, STORE_F, SpinF, STORE_H, SpinH)
   CPSHIP_XR(STORE_F, SpinF, STORE_H, SpinH, STORE_H, SpinH, STORE_H, SpinH)
   CPSHIP_XR(STORE_F, SpinF, STORE_F, SpinF, STORE_F, SpinF, STORE_F, SpinF)
+  CPSHIP_XR(STORE_H, SpinH, STORE_H, SpinH, STORE_H, SpinH, STORE_H, SpinH)
 #undef CPSHIP_XR
   return hipErrorInvalidValue;
 }

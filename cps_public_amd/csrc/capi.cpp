@@ -376,23 +376,28 @@ int cps_hip_invcg_device(cps_hip_lattice* lat, const cps_hip_op* op, double* sol
 // ---- MatInv ----------------------------------------------------------------
 // DiracOpWilson::MatInv (d_op_wilson.C:257-334) / DiracOpDwf::MatInv
 // (d_op_dwf.C:316-509): temp = in_o + kappa D in_e ; src = MatPcDag temp ;
-// InvCg ; out_e = in_e + kappa D out_o.  The source is preserved.
+// InvCg ; out_e = in_e + kappa D out_o.  DiracOpClover::MatInv and
+// DiracOpMobius::MatInv (ZMOB_PC_ORIG, d_op_mobius.C:389-640) add the clover
+// inverse / M5^-1 steps (clover.hip, mobius.hip).  Full vectors are
+// [odd][even] per rhs, except Moebius, which uses DWF_4D_EOPREC_EE
+// ([even][odd], d_op_mobius.C:410-415).  The source is preserved.
 int cps_hip_matinv(cps_hip_lattice* lat, const cps_hip_op* op, double* out, const double* in, int nrhs,
                    const cps_hip_cg_arg* arg, cps_hip_cg_result* res) {
   if (check_lat(lat) || !op || !out || !in || !arg || nrhs < 1) return CPS_HIP_ERR_ARG;
-  if (op->kind != CPS_HIP_OP_WILSON && op->kind != CPS_HIP_OP_DWF) return CPS_HIP_ERR_ARG;
   Lattice& L = lat->L;
   if (!L.has_gauge) return CPS_HIP_ERR_NOGAUGE;
   OpLayout o;
   int rc = op_layout(op, L.ls, &o, L.g);
   if (rc) return rc;
+  const bool mob = op->kind == CPS_HIP_OP_MOBIUS;
+  const int blk_odd = mob ? 1 : 0, blk_even = mob ? 0 : 1;   // block order of the full vector
   const int64_t half = o.per_rhs * 24;   // reals of one checkerboard of one rhs
   const int64_t n = o.per_rhs * nrhs;
   std::vector<double> h_odd(half * nrhs), h_even(half * nrhs), h_x(half * nrhs);
   for (int k = 0; k < nrhs; k++) {
-    std::memcpy(&h_odd[k * half], in + (2 * k) * half, half * sizeof(double));
-    std::memcpy(&h_even[k * half], in + (2 * k + 1) * half, half * sizeof(double));
-    std::memcpy(&h_x[k * half], out + (2 * k) * half, half * sizeof(double));
+    std::memcpy(&h_odd[k * half], in + (2 * k + blk_odd) * half, half * sizeof(double));
+    std::memcpy(&h_even[k * half], in + (2 * k + blk_even) * half, half * sizeof(double));
+    std::memcpy(&h_x[k * half], out + (2 * k + blk_odd) * half, half * sizeof(double));
   }
   Field& fo = L.field(S_IO_IN, STORE_D, n);
   Field& fe = L.field(S_EVEN_IN, STORE_D, n);
@@ -401,31 +406,65 @@ int cps_hip_matinv(cps_hip_lattice* lat, const cps_hip_op* op, double* out, cons
   Field& x = L.field(S_X, STORE_D, n);
   Field& t1 = L.field(S_T1, STORE_D, n);
   Field& t2 = L.field(S_T2, STORE_D, n);
+  Field& t3 = L.field(S_T3, STORE_D, n);
   Field& eo = L.field(S_EVEN_OUT, STORE_D, n);
   if (L.last_rc) return L.last_rc;
   if ((rc = upload(L, h_odd.data(), fo, o, 1, nrhs, 0))) return rc;
   if ((rc = upload(L, h_even.data(), fe, o, 0, nrhs, 1))) return rc;
   CPSHIP_CHECK(hipStreamSynchronize(L.stream));
+  if (mob) {
+    // the CG solves the 2 kappa_b-normalised system: scale the initial guess
+    const double g = op->mobius_b * (4 - op->m5) + op->a5_inv;
+    for (double& v : h_x) v *= g;
+  }
   if ((rc = upload(L, h_x.data(), x, o, 1, nrhs, 0))) return rc;
   const double kappa = op_kappa(op);
-  // temp = in_o + kappa * D(cb=0) in_e
-  if ((rc = hop(L, op, PREC_D_D18, tmp.ref(), fe.ref(), 1, 0, kappa, fo.ref(), 1.0, nullptr, nrhs))) return rc;
-  if ((rc = apply_matpc(L, op, PREC_D_D18, src.ref(), tmp.ref(), t1.ref(), t2.ref(), 1, nrhs))) return rc;
+  switch (op->kind) {
+    case CPS_HIP_OP_MOBIUS:
+      rc = mobius_matinv_prep(L, op, src.ref(), fo.ref(), fe.ref(), t1.ref(), t2.ref(), tmp.ref(), nrhs);
+      break;
+    case CPS_HIP_OP_CLOVER:
+      rc = clover_matinv_prep(L, op, src.ref(), fo.ref(), fe.ref(), t1.ref(), t2.ref(), tmp.ref(), nrhs);
+      break;
+    default:
+      // temp = in_o + kappa * D(cb=0) in_e ; src = MatPcDag temp
+      rc = hop(L, op, PREC_D_D18, tmp.ref(), fe.ref(), 1, 0, kappa, fo.ref(), 1.0, nullptr, nrhs);
+      if (!rc) rc = apply_matpc(L, op, PREC_D_D18, src.ref(), tmp.ref(), t1.ref(), t2.ref(), 1, nrhs);
+  }
+  if (rc) return rc;
   std::vector<int> iters, status, nup;
   std::vector<double> tr, sn;
   auto t0 = std::chrono::steady_clock::now();
   if (arg->precision == CPS_HIP_PREC_DOUBLE) rc = cg_double_device(L, op, x, src, nrhs, arg, iters, status, tr, sn);
   else rc = cg_mixed_device(L, op, x, src, nrhs, arg, iters, status, tr, sn, nup);
   if (rc) return rc;
-  // out_e = in_e + kappa * D(cb=1) out_o
-  if ((rc = hop(L, op, PREC_D_D18, eo.ref(), x.ref(), 0, 0, kappa, fe.ref(), 1.0, nullptr, nrhs))) return rc;
+  switch (op->kind) {
+    case CPS_HIP_OP_MOBIUS:
+      rc = mobius_matinv_reconstruct(L, op, eo.ref(), x.ref(), fe.ref(), t1.ref(), t2.ref(), t3.ref(), nrhs);
+      break;
+    case CPS_HIP_OP_CLOVER:
+      rc = clover_matinv_reconstruct(L, op, eo.ref(), x.ref(), fe.ref(), t1.ref(), t2.ref(), t3.ref(), nrhs);
+      break;
+    default:
+      // out_e = in_e + kappa * D(cb=1) out_o
+      rc = hop(L, op, PREC_D_D18, eo.ref(), x.ref(), 0, 0, kappa, fe.ref(), 1.0, nullptr, nrhs);
+  }
+  if (rc) return rc;
   CPSHIP_CHECK(hipStreamSynchronize(L.stream));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if ((rc = download(L, x, h_x.data(), o, 1, nrhs))) return rc;
   if ((rc = download(L, eo, h_even.data(), o, 0, nrhs))) return rc;
+  // Moebius: the preconditioned system is 2 kappa_b times the operator
+  // (d_op_mobius.C:628-633)
+  double fac = 1.0;
+  if (mob) fac = 1.0 / (op->mobius_b * (4 - op->m5) + op->a5_inv);
   for (int k = 0; k < nrhs; k++) {
-    std::memcpy(out + (2 * k) * half, &h_x[k * half], half * sizeof(double));
-    std::memcpy(out + (2 * k + 1) * half, &h_even[k * half], half * sizeof(double));
+    double* po = out + (2 * k + blk_odd) * half;
+    double* pe = out + (2 * k + blk_even) * half;
+    for (int64_t q = 0; q < half; q++) {
+      po[q] = fac * h_x[k * half + q];
+      pe[q] = fac * h_even[k * half + q];
+    }
     if (res) {
       res[k].iterations = iters[k] + 1;
       res[k].status = status[k];

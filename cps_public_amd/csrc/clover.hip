@@ -376,4 +376,66 @@ int clover_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& ou
   return clover_matpc(L, op, hprec, out, t2, t1, 1, nullptr, nrhs);
 }
 
+namespace {
+
+// out = A in for every site of a single-checkerboard field (A the clover set
+// of kappa index rhs / rhs_per_set at checkerboard cb)
+template <typename S>
+__global__ void __launch_bounds__(256) k_clover_apply(FieldRef out, FieldRef in, CloverRef c, int cb, int rhs_per_set,
+                                                      int Vcb) {
+  using R = typename SpinTraits<S>::R;
+  const int rhs = blockIdx.y;
+  const int site = blockIdx.x * blockDim.x + threadIdx.x;
+  if (site >= Vcb) return;
+  const int64_t e = (int64_t)rhs * Vcb + site;
+  const int set = rhs / rhs_per_set;
+  const int64_t c0 = (int64_t)(set * 2 + cb) * CloverOf<S>::nplanes * c.plane + site;
+  cplx<R> x[12], y[12];
+  spin_load<S, R>(in, e, x);
+  clover_mul<S, R>(c, c0, x, y);
+  spin_store<S, R>(out, e, y);
+}
+
+int clover_apply_d(Lattice& L, const FieldRef& out, const FieldRef& in, int cb, int rhs_per_set, int nrhs) {
+  CloverRef c;
+  c.data = L.clover[CLOVER_D].p;
+  c.plane = L.g.Vcb;
+  dim3 grid((unsigned)((L.g.Vcb + 255) / 256), (unsigned)nrhs), block(256);
+  hipLaunchKernelGGL(k_clover_apply<SpinD>, grid, block, 0, L.stream, out, in, c, cb, rhs_per_set, L.g.Vcb);
+  CPSHIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+// DiracOpClover::MatInv (d_op_clover.C, MatInv(Vector*, Vector*, Float*,
+// PreserveType)): full vectors [odd][even];
+//   t = in_o + kappa D(cb even) A_ee^-1 in_e ; src = MatPcDag t ; InvCg ;
+//   out_e = A_ee^-1 (in_e + kappa D(cb odd) out_o).
+// One kappa (the clover field must hold a single set).
+int clover_matinv_prep(Lattice& L, const cps_hip_op* op, const FieldRef& src, const FieldRef& in_o,
+                       const FieldRef& in_e, const FieldRef& t1, const FieldRef& t2, const FieldRef& t3, int nrhs) {
+  int per = 0;
+  int rc = clover_sets(L, op, nrhs, &per);
+  if (rc) return rc;
+  if (L.n_clover != 1) return CPS_HIP_ERR_ARG;
+  const double kappa = L.clover_kappas[0];
+  if ((rc = clover_apply_d(L, t3, in_e, 0, per, nrhs))) return rc;
+  if ((rc = hop(L, op, PREC_D_D18, t2, t3, 1, 0, kappa, in_o, 1.0, nullptr, nrhs))) return rc;
+  return clover_matpc(L, op, PREC_D_D18, src, t2, t1, 1, nullptr, nrhs);
+}
+
+int clover_matinv_reconstruct(Lattice& L, const cps_hip_op* op, const FieldRef& out_e, const FieldRef& out_o,
+                              const FieldRef& in_e, const FieldRef& t1, const FieldRef& t2, const FieldRef& t3,
+                              int nrhs) {
+  (void)t2;
+  (void)t3;
+  int per = 0;
+  int rc = clover_sets(L, op, nrhs, &per);
+  if (rc) return rc;
+  const double kappa = L.clover_kappas[0];
+  if ((rc = hop(L, op, PREC_D_D18, t1, out_o, 0, 0, kappa, in_e, 1.0, nullptr, nrhs))) return rc;
+  return clover_apply_d(L, out_e, t1, 0, per, nrhs);
+}
+
 }  // namespace cpship

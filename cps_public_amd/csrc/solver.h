@@ -183,6 +183,18 @@ int clover_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& ou
                  const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs);
 int clover_build(Lattice& L, const double* kappas, int n_kappa, double csw);
 int clover_export(Lattice& L, int k, int cb, double* A);
+// MatInv source preparation (src = MatPcDag of the prepared odd source) and
+// even-half reconstruction for the Moebius and clover operators
+int mobius_matinv_prep(Lattice& L, const cps_hip_op* op, const FieldRef& src, const FieldRef& in_o,
+                       const FieldRef& in_e, const FieldRef& t1, const FieldRef& t2, const FieldRef& t3, int nrhs);
+int mobius_matinv_reconstruct(Lattice& L, const cps_hip_op* op, const FieldRef& out_e, const FieldRef& out_o,
+                              const FieldRef& in_e, const FieldRef& t1, const FieldRef& t2, const FieldRef& t3,
+                              int nrhs);
+int clover_matinv_prep(Lattice& L, const cps_hip_op* op, const FieldRef& src, const FieldRef& in_o,
+                       const FieldRef& in_e, const FieldRef& t1, const FieldRef& t2, const FieldRef& t3, int nrhs);
+int clover_matinv_reconstruct(Lattice& L, const cps_hip_op* op, const FieldRef& out_e, const FieldRef& out_o,
+                              const FieldRef& in_e, const FieldRef& t1, const FieldRef& t2, const FieldRef& t3,
+                              int nrhs);
 
 // host <-> device field transfer with layout conversion; with on_device the
 // double pointer is device memory in the CPS layout (no PCIe copy)

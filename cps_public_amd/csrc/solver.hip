@@ -589,6 +589,15 @@ void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp
   }
 }
 
+static bool sloppy_vectors_half() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("CPS_HIP_SLOPPY_VECTORS");
+    v = (e && std::strcmp(e, "half") == 0) ? 1 : 0;
+  }
+  return v == 1;
+}
+
 static void mixed_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp_store, int* hprec) {
   precision_formats(prec, op, ls, nrhs, sp_store, hprec);
 }
@@ -639,8 +648,12 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   Field& ap = L.field(S_AP, STORE_D, n);
   Field& t1 = L.field(S_T1, STORE_D, n);
   Field& t2 = L.field(S_T2, STORE_D, n);
-  Field& xs = L.field(S_XS, STORE_F, n);
-  Field& rs = L.field(S_RS, STORE_F, n);
+  // sloppy solution and residual: FP32 by default; with half-precision
+  // sloppy storage and CPS_HIP_SLOPPY_VECTORS=half they are FP16 too (less
+  // BLAS traffic, more reliance on the reliable updates)
+  const int svec = (sps == STORE_H && sloppy_vectors_half()) ? STORE_H : STORE_F;
+  Field& xs = L.field(S_XS, svec, n);
+  Field& rs = L.field(S_RS, svec, n);
   Field& ps = L.field(S_PS, sps, n);
   Field& aps = L.field(S_APS, sps, n);
   Field& t1s = L.field(S_T1S, sps, n);
@@ -675,9 +688,9 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   if (rc) return rc;
   CPSHIP_CHECK(blas_sub_norm_d(src.ref(), ap.ref(), r.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
   CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, cg.s.rr_upd, RED_CG_INIT, cg.s, L.stream));
-  CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), STORE_F, rs.ref(), n, L.stream));
+  CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), svec, rs.ref(), n, L.stream));
   CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), sps, ps.ref(), n, L.stream));
-  CPSHIP_CHECK(blas_zero(STORE_F, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
+  CPSHIP_CHECK(blas_zero(svec, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
   {
     // the first iteration's direction update p = r + 0 p
     std::vector<double> zeros(nrhs, 0.0);
@@ -710,12 +723,12 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     // first, then Ap, alpha, x and r, |r|^2 and beta -- so that a reliable
     // update at a chunk boundary falls between r's update and p's
     rc = iter_graph.run(L, todo, [&]() -> int {
-      CPSHIP_CHECK(blas_xpay_mixed(sps, ps.ref(), STORE_F, rs.ref(), cg.s.beta, cg.s.active, o.per_rhs, nrhs, bsb,
+      CPSHIP_CHECK(blas_xpay_mixed(sps, ps.ref(), svec, rs.ref(), cg.s.beta, cg.s.active, o.per_rhs, nrhs, bsb,
                                    L.stream));
       int e = apply_mdagm(L, op, hprec, aps.ref(), ps.ref(), t1s.ref(), t2s.ref(), part, nrhs);
       if (e) return e;
       CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nrhs, nullptr, RED_CG_ALPHA, cg.s, L.stream));
-      CPSHIP_CHECK(blas_cg_xr_mixed(STORE_F, xs.ref(), sps, ps.ref(), STORE_F, rs.ref(), sps, aps.ref(), cg.s,
+      CPSHIP_CHECK(blas_cg_xr_mixed(svec, xs.ref(), sps, ps.ref(), svec, rs.ref(), sps, aps.ref(), cg.s,
                                     o.per_rhs, nrhs, bsb, part, L.stream));
       CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_CG_BETA, cg.s, L.stream));
       return 0;
@@ -741,20 +754,20 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     }
     if (since >= every || paused || done >= max_itr) {
       // ---- FP64-critical steps: accumulate the solution, recompute r ----
-      CPSHIP_CHECK(blas_axpy(STORE_F, xs.ref(), STORE_D, x.ref(), nullptr, 1.0, o.per_rhs, nrhs, bsb, L.stream));
-      CPSHIP_CHECK(blas_zero(STORE_F, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
+      CPSHIP_CHECK(blas_axpy(svec, xs.ref(), STORE_D, x.ref(), nullptr, 1.0, o.per_rhs, nrhs, bsb, L.stream));
+      CPSHIP_CHECK(blas_zero(svec, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
       rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), x.ref(), t1.ref(), t2.ref(), nullptr, nrhs);
       if (rc) return rc;
       CPSHIP_CHECK(blas_sub_norm_d(src.ref(), ap.ref(), r.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
       CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, rr_true, RED_PLAIN, cg.s, L.stream));
-      CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), STORE_F, rs.ref(), n, L.stream));
-      CPSHIP_CHECK(blas_redot_mixed(STORE_F, rs.ref(), sps, ps.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
+      CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), svec, rs.ref(), n, L.stream));
+      CPSHIP_CHECK(blas_redot_mixed(svec, rs.ref(), sps, ps.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
       CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, rp, RED_PLAIN, cg.s, L.stream));
       hipLaunchKernelGGL(k_mixed_after_update, dim3((nrhs + 63) / 64), dim3(64), 0, L.stream, cg.s, rr_true, rp, coef,
                          nrhs);
       CPSHIP_CHECK(hipGetLastError());
       // p -= (<r,p>/|r|^2) r for the continuing rhs (coef 0 for the others)
-      CPSHIP_CHECK(blas_axpy(STORE_F, rs.ref(), sps, ps.ref(), coef, 0.0, o.per_rhs, nrhs, bsb, L.stream));
+      CPSHIP_CHECK(blas_axpy(svec, rs.ref(), sps, ps.ref(), coef, 0.0, o.per_rhs, nrhs, bsb, L.stream));
       for (int k = 0; k < nrhs; k++)
         if (live[k]) n_updates[k]++;
       since = 0;

@@ -466,3 +466,68 @@ def test_half_storage_subnormal_site():
         assert np.isfinite(got).all()
         assert rel(got, ref) < SLOPPY_TOL[prec]
     lat.close()
+
+
+# ---------------------------------------------------------------------------
+# MatInv of the Moebius and clover operators against the full textbook
+# operators (tests/textbook.py)
+
+
+def _full_setup(L, seed):
+    from tests import textbook as T
+    from cps_public_amd import layout as LY
+    u = F.gauge_field(L, 0.2, seed)
+    H = T.hopping(T.links(u, L), L, antiperiodic=(0, 0, 0, 1))
+    uw = LY.gauge_canon_to_wilson(LY.bond_cond(u, L, [0, 0, 0, 1]), L)
+    return u, H, uw
+
+
+def test_mobius_matinv_solves_full_operator():
+    """DiracOpMobius::MatInv (ZMOB_PC_ORIG): out solves
+    [(b D_W + 1) + (c D_W - 1) P] out = in, vectors in DWF_4D_EOPREC_EE order."""
+    from tests import textbook as T
+    from tests.test_oracle_pins import prec4_from_canon, canon_from_prec4, rand_cplx
+    L, ls, mass, m5, b, c = [4, 4, 4, 4], 4, 0.05, 1.8, 1.5, 0.5
+    u, H, uw = _full_setup(L, 170)
+    D, _ = T.mobius_operator(H, ls, mass, m5, b, c)
+    kb, _ = T.mobius_kappas(m5, b, c)
+    V = int(np.prod(L))
+    rng = np.random.default_rng(171)
+    bc5 = rand_cplx(rng, ls, V, 12)
+    vin = np.concatenate([prec4_from_canon(bc5, L, ls, 0), prec4_from_canon(bc5, L, ls, 1)])
+    lat = C.Lattice(L, ls, (0, 0, 0, 1))
+    lat.import_gauge(uw)
+    op = C.make_op(C.OP_MOBIUS, mass=mass, m5=m5, b=b, c=c)
+    for prec in (C.PREC_DOUBLE, C.PREC_MIXED_HALF):
+        out, res = lat.matinv(op, vin, C.make_cg_arg(1e-12, 5000, precision=prec))
+        h = len(out) // 2
+        x = (canon_from_prec4(out[:h], L, ls, 0) + canon_from_prec4(out[h:], L, ls, 1)).reshape(-1)
+        r = D @ x / (2 * kb) - bc5.reshape(-1)
+        print("mobius matinv", prec, res[0].iterations, np.linalg.norm(r) / np.linalg.norm(bc5), flush=True)
+        assert np.linalg.norm(r) / np.linalg.norm(bc5) < 1e-8
+    lat.close()
+
+
+def test_clover_matinv_solves_full_operator():
+    """DiracOpClover::MatInv: out solves (A - kappa H) out = in, [odd][even]."""
+    from tests import textbook as T
+    from tests.test_oracle_pins import full_from_canon, canon_from_full, rand_cplx
+    from cps_public_amd import layout as LY
+    L, kappa, csw = [4, 4, 4, 8], 0.13, 1.2
+    u, H, uw = _full_setup(L, 180)
+    Ub = T.links(LY.bond_cond(u, L, [0, 0, 0, 1]), L)
+    D = (T.block_diag_sparse(T.clover_blocks(Ub, L, kappa, csw)) - kappa * H).tocsr()
+    V = int(np.prod(L))
+    rng = np.random.default_rng(181)
+    bvec = rand_cplx(rng, V, 12)
+    lat = C.Lattice(L, 1, (0, 0, 0, 1))
+    lat.import_gauge(uw)
+    lat.clover_build([kappa], csw)
+    op = C.make_op(C.OP_CLOVER, mass=1.0 / (2 * kappa) - 4.0, csw=csw)
+    for prec in (C.PREC_DOUBLE, C.PREC_MIXED_SINGLE):
+        out, res = lat.matinv(op, full_from_canon(bvec, L), C.make_cg_arg(1e-12, 5000, precision=prec))
+        x = canon_from_full(out, L).reshape(-1)
+        r = D @ x - bvec.reshape(-1)
+        print("clover matinv", prec, res[0].iterations, np.linalg.norm(r) / np.linalg.norm(bvec), flush=True)
+        assert np.linalg.norm(r) / np.linalg.norm(bvec) < 1e-8
+    lat.close()
