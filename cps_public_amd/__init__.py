@@ -89,6 +89,7 @@ def _declare(l):
         "cps_hip_invcg": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_invcg_device": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_matinv": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
+        "cps_hip_qpropw_point": [vp, P(Op), P(ctypes.c_int), P(CgArg), vp, P(CgResult)],
         "cps_hip_bench_hop": [vp, i, i, i, i, P(d), P(d)],
         "cps_hip_bench_copy": [vp, ctypes.c_longlong, i, P(d)],
         "cps_hip_device_count": [],
@@ -256,6 +257,21 @@ class Lattice:
         _check(lib().cps_hip_matinv(self.h, ctypes.byref(op), _ptr(out), _ptr(b), nrhs, ctypes.byref(arg), res),
                "cps_hip_matinv")
         return out, [res[k] for k in range(nrhs)]
+
+    def qpropw_point(self, op, x0, arg):
+        """QPropW::Run for a point source at x0: the 12 spin-colour solves as
+        one batched MatInv; returns the propagator as an array
+        [t][z][y][x][s1][c1][spin][colour] of complex (sink s1, c1; source
+        spin, colour) and the 12 solve results (rhs = 3 spin + colour)."""
+        V = int(np.prod(self.L))
+        prop = np.zeros(V * 288)
+        res = (CgResult * 12)()
+        xs = (ctypes.c_int * 4)(*[int(v) for v in x0])
+        _check(lib().cps_hip_qpropw_point(self.h, ctypes.byref(op), xs, ctypes.byref(arg), _ptr(prop), res),
+               "cps_hip_qpropw_point")
+        p = prop.reshape(V, 4, 3, 4, 3, 2)
+        z = (p[..., 0] + 1j * p[..., 1]).reshape(self.L[3], self.L[2], self.L[1], self.L[0], 4, 3, 4, 3)
+        return z, [res[k] for k in range(12)]
 
     def bench_hop(self, kind, precision, nrhs=1, reps=20):
         s, b = ctypes.c_double(), ctypes.c_double()

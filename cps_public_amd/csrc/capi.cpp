@@ -401,12 +401,7 @@ int cps_hip_matinv(cps_hip_lattice* lat, const cps_hip_op* op, double* out, cons
   }
   Field& fo = L.field(S_IO_IN, STORE_D, n);
   Field& fe = L.field(S_EVEN_IN, STORE_D, n);
-  Field& tmp = L.field(S_TMP, STORE_D, n);
-  Field& src = L.field(S_SRC, STORE_D, n);
   Field& x = L.field(S_X, STORE_D, n);
-  Field& t1 = L.field(S_T1, STORE_D, n);
-  Field& t2 = L.field(S_T2, STORE_D, n);
-  Field& t3 = L.field(S_T3, STORE_D, n);
   Field& eo = L.field(S_EVEN_OUT, STORE_D, n);
   if (L.last_rc) return L.last_rc;
   if ((rc = upload(L, h_odd.data(), fo, o, 1, nrhs, 0))) return rc;
@@ -418,60 +413,18 @@ int cps_hip_matinv(cps_hip_lattice* lat, const cps_hip_op* op, double* out, cons
     for (double& v : h_x) v *= g;
   }
   if ((rc = upload(L, h_x.data(), x, o, 1, nrhs, 0))) return rc;
-  const double kappa = op_kappa(op);
-  switch (op->kind) {
-    case CPS_HIP_OP_MOBIUS:
-      rc = mobius_matinv_prep(L, op, src.ref(), fo.ref(), fe.ref(), t1.ref(), t2.ref(), tmp.ref(), nrhs);
-      break;
-    case CPS_HIP_OP_CLOVER:
-      rc = clover_matinv_prep(L, op, src.ref(), fo.ref(), fe.ref(), t1.ref(), t2.ref(), tmp.ref(), nrhs);
-      break;
-    default:
-      // temp = in_o + kappa * D(cb=0) in_e ; src = MatPcDag temp
-      rc = hop(L, op, PREC_D_D18, tmp.ref(), fe.ref(), 1, 0, kappa, fo.ref(), 1.0, nullptr, nrhs);
-      if (!rc) rc = apply_matpc(L, op, PREC_D_D18, src.ref(), tmp.ref(), t1.ref(), t2.ref(), 1, nrhs);
-  }
-  if (rc) return rc;
-  std::vector<int> iters, status, nup;
-  std::vector<double> tr, sn;
-  auto t0 = std::chrono::steady_clock::now();
-  if (arg->precision == CPS_HIP_PREC_DOUBLE) rc = cg_double_device(L, op, x, src, nrhs, arg, iters, status, tr, sn);
-  else rc = cg_mixed_device(L, op, x, src, nrhs, arg, iters, status, tr, sn, nup);
-  if (rc) return rc;
-  switch (op->kind) {
-    case CPS_HIP_OP_MOBIUS:
-      rc = mobius_matinv_reconstruct(L, op, eo.ref(), x.ref(), fe.ref(), t1.ref(), t2.ref(), t3.ref(), nrhs);
-      break;
-    case CPS_HIP_OP_CLOVER:
-      rc = clover_matinv_reconstruct(L, op, eo.ref(), x.ref(), fe.ref(), t1.ref(), t2.ref(), t3.ref(), nrhs);
-      break;
-    default:
-      // out_e = in_e + kappa * D(cb=1) out_o
-      rc = hop(L, op, PREC_D_D18, eo.ref(), x.ref(), 0, 0, kappa, fe.ref(), 1.0, nullptr, nrhs);
-  }
-  if (rc) return rc;
-  CPSHIP_CHECK(hipStreamSynchronize(L.stream));
-  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if ((rc = matinv_device(L, op, fo, fe, x, eo, nrhs, arg, res))) return rc;
   if ((rc = download(L, x, h_x.data(), o, 1, nrhs))) return rc;
   if ((rc = download(L, eo, h_even.data(), o, 0, nrhs))) return rc;
   // Moebius: the preconditioned system is 2 kappa_b times the operator
   // (d_op_mobius.C:628-633)
-  double fac = 1.0;
-  if (mob) fac = 1.0 / (op->mobius_b * (4 - op->m5) + op->a5_inv);
+  const double fac = mob ? 1.0 / (op->mobius_b * (4 - op->m5) + op->a5_inv) : 1.0;
   for (int k = 0; k < nrhs; k++) {
     double* po = out + (2 * k + blk_odd) * half;
     double* pe = out + (2 * k + blk_even) * half;
     for (int64_t q = 0; q < half; q++) {
       po[q] = fac * h_x[k * half + q];
       pe[q] = fac * h_even[k * half + q];
-    }
-    if (res) {
-      res[k].iterations = iters[k] + 1;
-      res[k].status = status[k];
-      res[k].true_res = tr[k];
-      res[k].src_norm_sq = sn[k];
-      res[k].reliable_updates = nup.empty() ? 0 : nup[k];
-      res[k].seconds = secs;
     }
   }
   return 0;

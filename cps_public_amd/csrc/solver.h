@@ -196,6 +196,13 @@ int clover_matinv_reconstruct(Lattice& L, const cps_hip_op* op, const FieldRef& 
                               const FieldRef& in_e, const FieldRef& t1, const FieldRef& t2, const FieldRef& t3,
                               int nrhs);
 
+// MatInv on device fields: in_o / in_e the source checkerboards (odd =
+// the CG's checkerboard), x the initial guess and on return the odd
+// solution, out_e the even solution; Moebius results are 2 kappa_b times the
+// DiracOpMobius::MatInv ones (the caller scales)
+int matinv_device(Lattice& L, const cps_hip_op* op, Field& in_o, Field& in_e, Field& x, Field& out_e, int nrhs,
+                  const cps_hip_cg_arg* arg, cps_hip_cg_result* res);
+
 // host <-> device field transfer with layout conversion; with on_device the
 // double pointer is device memory in the CPS layout (no PCIe copy)
 int upload(Lattice& L, const double* host, Field& f, const OpLayout& o, int cb5, int nrhs, int slot_io,

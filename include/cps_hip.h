@@ -166,11 +166,29 @@ int cps_hip_invcg(cps_hip_lattice *lat, const cps_hip_op *op, double *sol, const
 int cps_hip_invcg_device(cps_hip_lattice *lat, const cps_hip_op *op, double *sol, const double *src, int nrhs,
                          const cps_hip_cg_arg *arg, cps_hip_cg_result *res);
 
-/* ---- MatInv (d_op_wilson.C:257-334, d_op_dwf.C:316-509) ------------------
+/* ---- MatInv (d_op_wilson.C:257-334, d_op_dwf.C:316-509,
+ * d_op_mobius.C:389-640, d_op_clover.C:280-352) ----------------------------
  * Full-vector solve M out = in by even-odd preconditioning + InvCg; in and
- * out are full WILSON-order vectors ([odd][even]).  in is preserved. */
+ * out are full vectors in the DiracOp's order: [odd][even] (WILSON, DWF),
+ * [even][odd] for Moebius (DWF_4D_EOPREC_EE, d_op_mobius.C:410-415).  out
+ * holds the initial guess on entry.  in is preserved.  Clover needs exactly
+ * one kappa set built (cps_hip_clover_build). */
 int cps_hip_matinv(cps_hip_lattice *lat, const cps_hip_op *op, double *out, const double *in, int nrhs,
                    const cps_hip_cg_arg *arg, cps_hip_cg_result *res);
+
+/* ---- point-source propagator (QPropW::Run, QPropW.C:566-613) -------------
+ * The 12 spin-colour point sources at site x0 (FermionVectorTp::
+ * SetPointSource, FermionVector.C:1567-1644; for DWF/Moebius placed at
+ * s = 0 / Ls-1 by Ffour2five, f_dwf_base.C:961-1050), solved as ONE batched
+ * MatInv (QPropW::CG, QPropW.C:1077-1267; Moebius multiplies the source by
+ * D_minus first, f_mobius.C:68-78), the 4-D solution taken by Ffive2four
+ * (f_dwf_base.C:1068-1168) and stored as WilsonMatrix columns (LoadRow,
+ * QPropW.C:1343-1364): prop[site][s1][c1][spin][colour][re,im], 288 doubles
+ * per CANONICAL site x + Lx(y + Ly(z + Lz t)).  res receives the 12 solve
+ * results (rhs k = 3 spin + colour).  Gauge fixing of the sink (FixSol) and
+ * the midpoint propagator are not part of it. */
+int cps_hip_qpropw_point(cps_hip_lattice *lat, const cps_hip_op *op, const int x0[4], const cps_hip_cg_arg *arg,
+                         double *prop, cps_hip_cg_result *res);
 
 /* ---- measurement helpers ------------------------------------------------ */
 /* Time `reps` applications of the single-parity hop kernel of an operator

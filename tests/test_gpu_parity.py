@@ -531,3 +531,87 @@ def test_clover_matinv_solves_full_operator():
         print("clover matinv", prec, res[0].iterations, np.linalg.norm(r) / np.linalg.norm(bvec), flush=True)
         assert np.linalg.norm(r) / np.linalg.norm(bvec) < 1e-8
     lat.close()
+
+
+# ---------------------------------------------------------------------------
+# batched point-source propagator (QPropW::Run's 12 spin-colour solves)
+
+
+def _four2five(b4, ls):
+    """Ffour2five(src_5d, src_4d, 0, Ls-1): spins 0,1 at s = 0, spins 2,3 at
+    s = Ls-1 (canonical (V, 12) -> (Ls, V, 12))."""
+    b5 = np.zeros((ls,) + b4.shape, dtype=complex)
+    b5[0, :, :6] = b4[:, :6]
+    b5[ls - 1, :, 6:] = b4[:, 6:]
+    return b5
+
+
+def _five2four(x5):
+    """Ffive2four(sol_4d, sol_5d, Ls-1, 0): spins 0,1 from s = Ls-1, spins
+    2,3 from s = 0."""
+    x4 = np.zeros(x5.shape[1:], dtype=complex)
+    x4[:, :6] = x5[-1, :, :6]
+    x4[:, 6:] = x5[0, :, 6:]
+    return x4
+
+
+@pytest.mark.parametrize("kind,x0,prec", [
+    ("wilson", (0, 0, 0, 0), C.PREC_DOUBLE), ("wilson", (1, 2, 3, 1), C.PREC_DOUBLE),
+    ("clover", (1, 2, 3, 1), C.PREC_DOUBLE), ("dwf", (1, 2, 3, 1), C.PREC_DOUBLE),
+    ("dwf", (2, 0, 1, 3), C.PREC_MIXED_HALF), ("mobius", (1, 2, 3, 1), C.PREC_DOUBLE),
+    ("mobius", (0, 1, 1, 0), C.PREC_MIXED_HALF)])
+def test_qpropw_point_propagator(kind, x0, prec):
+    """cps_hip_qpropw_point against the full textbook operators: column
+    (spin, colour) of the propagator is the 4-D projection of D^-1 applied
+    to the point source (D_minus first for Moebius), for all 12 columns."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from tests import textbook as T
+    from cps_public_amd import layout as LY
+    L = [4, 4, 4, 4]
+    V = int(np.prod(L))
+    ls = 1 if kind in ("wilson", "clover") else 4
+    u, H, uw = _full_setup(L, 190)
+    mass, m5, b, c = 0.1, 1.8, 1.5, 0.5
+    lat = C.Lattice(L, ls, (0, 0, 0, 1))
+    lat.import_gauge(uw)
+    if kind == "wilson":
+        op = C.make_op(C.OP_WILSON, mass=mass)
+        D = sp.identity(12 * V) - (1.0 / (2 * (mass + 4))) * H
+    elif kind == "clover":
+        kappa, csw = 0.12, 1.1
+        lat.clover_build([kappa], csw)
+        op = C.make_op(C.OP_CLOVER, mass=1.0 / (2 * kappa) - 4.0, csw=csw)
+        Ub = T.links(LY.bond_cond(u, L, [0, 0, 0, 1]), L)
+        D = T.block_diag_sparse(T.clover_blocks(Ub, L, kappa, csw)) - kappa * H
+    elif kind == "dwf":
+        op = C.make_op(C.OP_DWF, mass=mass, m5=m5)
+        D = T.dwf_operator(H, ls, mass, m5)[0]
+    else:
+        op = C.make_op(C.OP_MOBIUS, mass=mass, m5=m5, b=b, c=c)
+        Dm, _ = T.mobius_operator(H, ls, mass, m5, b, c)
+        kb, _ = T.mobius_kappas(m5, b, c)
+        D = Dm / (2 * kb)
+        Dminus = (1 - c * (4 - m5)) * sp.identity(12 * V * ls) + 0.5 * c * sp.kron(sp.identity(ls), H)
+    lu = spla.splu(sp.csc_matrix(D))
+    prop, res = lat.qpropw_point(op, x0, C.make_cg_arg(1e-12, 5000, precision=prec))
+    lat.close()
+    site = x0[0] + L[0] * (x0[1] + L[1] * (x0[2] + L[2] * x0[3]))
+    got = prop.reshape(V, 12, 12)
+    worst = 0.0
+    for spin in range(4):
+        for col in range(3):
+            b4 = np.zeros((V, 12), dtype=complex)
+            b4[site, 3 * spin + col] = 1.0
+            if ls == 1:
+                x4 = lu.solve(b4.reshape(-1)).reshape(V, 12)
+            else:
+                b5 = _four2five(b4, ls).reshape(-1)
+                if kind == "mobius":
+                    b5 = Dminus @ b5
+                x4 = _five2four(lu.solve(b5).reshape(ls, V, 12))
+            col_got = got[:, :, 3 * spin + col]
+            worst = max(worst, np.linalg.norm(col_got - x4) / np.linalg.norm(x4))
+    print("qpropw", kind, x0, prec, [r.iterations for r in res], worst, flush=True)
+    assert max(r.true_res for r in res) <= 1e-10
+    assert worst < (1e-9 if prec == C.PREC_DOUBLE else 1e-8)
