@@ -258,6 +258,182 @@ __global__ void k_m5inv(M5Params P) {
   }
 }
 
+// The same recurrence with Ls a compile-time constant: the Ls values of one
+// complex component live in registers (fully unrolled, no LDS round trip
+// per step) and the divisions by -K become products with 1/(-K).
+template <typename R, int LS, int DAG, int LOWER>
+__device__ __forceinline__ void m5inv_regs(cplx<R> (&v)[LS], R K, R mass) {
+  const R mK = -K, mKinv = R(1) / (-K);
+  if constexpr (DAG == 0) {
+    R prod = mass;
+#pragma unroll
+    for (int i = 0; i < LS; i++) prod *= mK;
+    const R inv_d_last = R(1) / (R(1) + prod);
+    if constexpr (!LOWER) {
+#pragma unroll
+      for (int s = 0; s <= LS - 2; ++s) {
+        v[s + 1].re += mK * v[s].re;
+        v[s + 1].im += mK * v[s].im;
+      }
+      R prd = 1;
+#pragma unroll
+      for (int i = 0; i < LS - 1; i++) prd *= mK;
+      R f = -prd * mass * inv_d_last;
+#pragma unroll
+      for (int s = LS - 2; s >= 0; --s) {
+        v[s].re += f * v[LS - 1].re;
+        v[s].im += f * v[LS - 1].im;
+        f *= mKinv;
+      }
+      v[LS - 1].re *= inv_d_last;
+      v[LS - 1].im *= inv_d_last;
+    } else {
+      v[LS - 1].re *= inv_d_last;
+      v[LS - 1].im *= inv_d_last;
+      R f = K * mass * inv_d_last;
+#pragma unroll
+      for (int s = 0; s <= LS - 2; ++s) {
+        v[LS - 1].re += f * v[s].re;
+        v[LS - 1].im += f * v[s].im;
+        f *= mK;
+      }
+#pragma unroll
+      for (int s = LS - 2; s >= 0; --s) {
+        v[s].re += mK * v[s + 1].re;
+        v[s].im += mK * v[s + 1].im;
+      }
+    }
+  } else {
+    R prod = mass;
+#pragma unroll
+    for (int i = 0; i < LS; i++) prod *= K;
+    const R inv_d_last = R(1) / (R(1) + prod);
+    if constexpr (!LOWER) {
+      v[LS - 1].re *= inv_d_last;
+      v[LS - 1].im *= inv_d_last;
+      R f = K * mass * inv_d_last;
+#pragma unroll
+      for (int s = 0; s <= LS - 2; ++s) {
+        v[LS - 1].re += f * v[s].re;
+        v[LS - 1].im += f * v[s].im;
+        f *= mK;
+      }
+#pragma unroll
+      for (int s = LS - 2; s >= 0; --s) {
+        v[s].re += mK * v[s + 1].re;
+        v[s].im += mK * v[s + 1].im;
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s <= LS - 2; ++s) {
+        v[s + 1].re += mK * v[s].re;
+        v[s + 1].im += mK * v[s].im;
+      }
+      R prd = 1;
+#pragma unroll
+      for (int i = 0; i < LS - 1; i++) prd *= mK;
+      R f = -prd * mass * inv_d_last;
+#pragma unroll
+      for (int s = LS - 2; s >= 0; --s) {
+        v[s].re += f * v[LS - 1].re;
+        v[s].im += f * v[LS - 1].im;
+        if (s > 0) f *= mKinv;
+      }
+      v[LS - 1].re *= inv_d_last;
+      v[LS - 1].im *= inv_d_last;
+    }
+  }
+}
+
+// M5^-1 for a compile-time Ls.  A block stages M5_SITES sites x LS slots in
+// LDS (per-site stride padded by 4 reals against bank conflicts); phase 2
+// gives each (site, complex component) pair to one thread, which runs the
+// recurrence on registers.
+template <int LS> struct M5Sites { static constexpr int value = LS <= 16 ? 256 / 16 : 256 / LS; };
+
+template <typename S, int DAG, int LS>
+__global__ void __launch_bounds__(256) k_m5inv_t(M5Params P) {
+  using R = typename SpinTraits<S>::R;
+  constexpr int SITES = M5Sites<LS>::value;
+  constexpr int STRIDE = LS * 24 + 4;   // reals per site in LDS
+  __shared__ R lds[SITES * STRIDE];
+  const int64_t site0 = (int64_t)blockIdx.x * SITES;
+  const int sl_t = threadIdx.x / LS, s_t = threadIdx.x - (threadIdx.x / LS) * LS;
+  const int64_t site_t = site0 + sl_t;
+  const bool mine = sl_t < SITES && site_t < P.nsites;
+  if (mine) {
+    cplx<R> v[12];
+    spin_load<S, R>(P.f, site_t * LS + s_t, v);
+    R* d = lds + sl_t * STRIDE + s_t * 24;
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      d[2 * c] = v[c].re;
+      d[2 * c + 1] = v[c].im;
+    }
+  }
+  __syncthreads();
+  if (P.s5 == S5_BEFORE) {
+    R t[24];
+    if (mine) s5_lds_t<R, DAG>(lds + sl_t * STRIDE, s_t, LS, R(P.mass), R(P.b5), R(P.c5), t);
+    __syncthreads();
+    if (mine) {
+      R* d = lds + sl_t * STRIDE + s_t * 24;
+#pragma unroll
+      for (int k = 0; k < 24; k++) d[k] = t[k];
+    }
+    __syncthreads();
+  }
+  // phase 2: (site, complex component c) -> registers; components 0-5 are
+  // spins 0,1 (upper), 6-11 spins 2,3 (lower)
+  for (int w = threadIdx.x; w < SITES * 12; w += blockDim.x) {
+    const int sl = w / 12, c = w - sl * 12;
+    if (site0 + sl < P.nsites) {
+      R* col = lds + sl * STRIDE + 2 * c;
+      cplx<R> v[LS];
+#pragma unroll
+      for (int s = 0; s < LS; s++) v[s] = cmk<R>(col[s * 24], col[s * 24 + 1]);
+      if (c < 6) m5inv_regs<R, LS, DAG, 0>(v, R(P.K), R(P.mass));
+      else m5inv_regs<R, LS, DAG, 1>(v, R(P.K), R(P.mass));
+#pragma unroll
+      for (int s = 0; s < LS; s++) {
+        col[s * 24] = v[s].re;
+        col[s * 24 + 1] = v[s].im;
+      }
+    }
+  }
+  __syncthreads();
+  if (mine) {
+    cplx<R> v[12];
+    if (P.s5 == S5_AFTER) {
+      R t[24];
+      s5_lds_t<R, DAG>(lds + sl_t * STRIDE, s_t, LS, R(P.mass), R(P.b5), R(P.c5), t);
+#pragma unroll
+      for (int c = 0; c < 12; c++) v[c] = cmk<R>(t[2 * c], t[2 * c + 1]);
+    } else {
+      const R* d = lds + sl_t * STRIDE + s_t * 24;
+#pragma unroll
+      for (int c = 0; c < 12; c++) v[c] = cmk<R>(d[2 * c], d[2 * c + 1]);
+    }
+    spin_store<S, R>(P.out, site_t * LS + s_t, v);
+  }
+}
+
+template <int LS>
+static bool launch_m5inv_t(const M5Params& P, int st, int dag, hipStream_t stream) {
+  constexpr int SITES = M5Sites<LS>::value;
+  dim3 grid((unsigned)((P.nsites + SITES - 1) / SITES)), block(SITES * LS);
+  switch (st) {
+#define CPSHIP_M5T(S)                                                                  \
+  if (dag) hipLaunchKernelGGL((k_m5inv_t<S, 1, LS>), grid, block, 0, stream, P);       \
+  else hipLaunchKernelGGL((k_m5inv_t<S, 0, LS>), grid, block, 0, stream, P);
+    case STORE_D: CPSHIP_M5T(SpinD) break;
+    case STORE_F: CPSHIP_M5T(SpinF) break;
+    default: CPSHIP_M5T(SpinH) break;
+#undef CPSHIP_M5T
+  }
+  return true;
+}
+
 static int store_of(int hprec) { return hprec == PREC_D_D18 ? STORE_D : hprec == PREC_F_F12 ? STORE_F : STORE_H; }
 
 static int launch_s5(Lattice& L, int hprec, S5Params& P) {
@@ -289,9 +465,27 @@ static int launch_m5inv(Lattice& L, int hprec, const FieldRef& f, int dag, doubl
   P.K = K;
   P.mass = mass;
   P.nsites = (int64_t)nrhs * L.g.Vcb;
+  const int st = store_of(hprec);
+  bool done = false;
+  switch (L.ls) {
+    case 4: done = launch_m5inv_t<4>(P, st, dag, L.stream); break;
+    case 6: done = launch_m5inv_t<6>(P, st, dag, L.stream); break;
+    case 8: done = launch_m5inv_t<8>(P, st, dag, L.stream); break;
+    case 10: done = launch_m5inv_t<10>(P, st, dag, L.stream); break;
+    case 12: done = launch_m5inv_t<12>(P, st, dag, L.stream); break;
+    case 16: done = launch_m5inv_t<16>(P, st, dag, L.stream); break;
+    case 20: done = launch_m5inv_t<20>(P, st, dag, L.stream); break;
+    case 24: done = launch_m5inv_t<24>(P, st, dag, L.stream); break;
+    case 32: done = launch_m5inv_t<32>(P, st, dag, L.stream); break;
+    default: break;
+  }
+  if (done) {
+    CPSHIP_CHECK(hipGetLastError());
+    return 0;
+  }
+  // other Ls: the generic kernel (Ls at run time)
   const int sites_per_block = std::max(1, 256 / L.ls);
   const int bs = sites_per_block * L.ls;
-  const int st = store_of(hprec);
   const size_t rbytes = st == STORE_D ? sizeof(double) : sizeof(float);
   const size_t lds = (size_t)sites_per_block * L.ls * 24 * rbytes;
   dim3 grid((unsigned)((P.nsites + sites_per_block - 1) / sites_per_block)), block(bs);
