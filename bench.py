@@ -9,13 +9,13 @@ sources resident in HBM (FP64, CPS layout) to solutions resident in HBM with
 their FP64 true residual computed; the gauge field is resident beforehand.
 
 Workloads (--config):
-  c2 (default)  Shamir DWF 16^3x32, Ls=16, M5=1.8, mf=0.01, links exp(i 0.2 H),
-                one Gaussian source; mixed = FP32 math, FP16 spinors and links,
-                FP64 reliable updates at least every 100 iterations.
-  c3            Moebius DWF 24^3x64, Ls=16, b=1.5, c=0.5, M5=1.8, mf=0.005,
-                12 point sources (spin x colour) in one batched solve, BF16
-                links, FP16 spinors.
-  c3h           c3 with FP16 links.
+  c3h (default) Moebius DWF 24^3x64, Ls=16, b=1.5, c=0.5, M5=1.8, mf=0.005,
+                links exp(i 0.2 H), 12 point sources (spin x colour) in one
+                batched solve; mixed = FP32 math, FP16 spinors and links, FP64
+                reliable updates at least every 100 iterations.
+  c3            c3h with BF16 instead of FP16 link storage.
+  c2            Shamir DWF 16^3x32, Ls=16, M5=1.8, mf=0.01, links exp(i 0.2 H),
+                one Gaussian source, mixed as c3h.
   c4            clover 32^3x64, c_sw 1.0, links exp(i 0.15 H), 12 point sources
                 x 4 kappas (0.1360-0.1375) = 48 rhs in one batched solve, FP16
                 spinors and links.
@@ -119,6 +119,27 @@ def cpu_baseline(cfg, u_w, src, gpu_iters):
     per = 24 * (int(np.prod(L)) // 2) * ls
     s0 = np.ascontiguousarray(src[:per])
     n_it = 3 if ls > 1 or np.prod(L) > 65536 else 50
+    big = np.prod(L) * ls > 4_000_000
+    if big and cfg["kind"] in (C.OP_MOBIUS, C.OP_DWF):
+        # one M^dag M of the oracle takes ~100 s single-threaded at this size:
+        # time it on a sub-lattice with Lt/8 time slices (same Ls and
+        # parameters, cost linear in the volume) and scale to the workload
+        Lt_sub = max(2, L[3] // 8)
+        Ls_ = [L[0], L[1], L[2], Lt_sub]
+        u_sub = LY.gauge_canon_to_wilson(LY.bond_cond(F.gauge_field(Ls_, cfg["eps"], cfg["seed"]), Ls_, [0, 0, 0, 1]),
+                                         Ls_)
+        s_sub = F.gaussian_fermion(int(np.prod(Ls_)) // 2 * ls, cfg["seed"] + 1)
+        t0 = time.perf_counter()
+        if cfg["kind"] == C.OP_MOBIUS:
+            O.mobius_mdagm(u_sub, s_sub, cfg["mass"], O.mobius_struct(Ls_, ls, cfg["m5"], cfg["b"], cfg["c"]))
+        else:
+            O.dwf_mdagm(u_sub, s_sub, cfg["mass"], O.dwf_struct(Ls_, ls, cfg["m5"]))
+        el = (time.perf_counter() - t0) * L[3] / Lt_sub
+        return {"value": el * gpu_iters, "unit": "s/RHS", "cores": 1, "kind": "port",
+                "sample": f"one M^dag M of the oracle (FP64, one thread) on a {L[0]}x{L[1]}x{L[2]}x{Lt_sub} "
+                          f"sub-lattice with Ls {ls} and the same parameters, "
+                          f"{el * Lt_sub / L[3]:.1f} s, x {L[3] // Lt_sub} (volume) = {el:.1f} s per M^dag M; "
+                          f"x {gpu_iters} iterations (the GPU FP64 count; CG vector updates not counted)"}
     if cfg["kind"] == C.OP_CLOVER:
         # the clover matrices are set-up work (CloverMatChkb), outside the clock
         kappa = cfg["kappas"][0]
@@ -226,9 +247,9 @@ def serve_stream(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c5"])
+    ap.add_argument("--config", default="c3h", choices=sorted(CONFIGS) + ["c5"])
     ap.add_argument("--requests", type=int, default=48, help="c5: number of requests in the stream")
     ap.add_argument("--rate", type=float, default=1.0, help="c5: Poisson arrival rate (requests/s)")
     ap.add_argument("--full", action="store_true",
@@ -316,10 +337,14 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 math / f16 spinors, f64 reliable updates" if mixed is not None else "f64",
+        "dtype": "f32" if mixed is not None else "f64",
         "data": "synthetic (seeded SU(3) links exp(i eps H), Philox)",
         "config": {"workload": cfg["name"], "nrhs": nrhs, "stop_rsd": 1e-10,
-                   "precision": {0: "fp64", 1: "mixed_fp16", 2: "mixed_bf16links", 3: "mixed_fp32"}[prec_main]},
+                   "precision": {0: "fp64", 1: "mixed_fp16", 2: "mixed_bf16links", 3: "mixed_fp32"}[prec_main],
+                   "storage": {0: "f64 spinors and links",
+                               1: "f16 spinors (per-site norms) + f16 links, f32 CG vectors, f64 reliable updates",
+                               2: "f16 spinors (per-site norms) + bf16 links, f32 CG vectors, f64 reliable updates",
+                               3: "f32 spinors + f32 links, f64 reliable updates"}[prec_main]},
         "iterations": [r.iterations for r in res_main],
         "true_res_max": max(r.true_res for r in res_main),
         "reliable_updates": [r.reliable_updates for r in res_main],

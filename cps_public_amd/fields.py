@@ -20,19 +20,37 @@ def _rng(seed, stream=0):
     return np.random.Generator(np.random.Philox(key=[int(seed) & (2**64 - 1), int(stream)]))
 
 
-def _expm_i_eps_h(h, eps, order=14, squarings=5):
-    """exp(i*eps*h) for a stack of 3x3 Hermitian matrices."""
-    a = (1j * eps / (2 ** squarings)) * h
+def _mul3(a, b):
+    """Product of two stacks of 3x3 matrices held as [row][col] lists of
+    arrays (structure of arrays: one vector per matrix element)."""
+    return [[a[i][0] * b[0][j] + a[i][1] * b[1][j] + a[i][2] * b[2][j] for j in range(3)] for i in range(3)]
+
+
+def _expm_i_eps_h(h, eps, order=10, squarings=5, chunk=1 << 16):
+    """exp(i*eps*h) for a stack of 3x3 Hermitian matrices: Taylor series of
+    order `order` after `squarings` halvings, then repeated squaring
+    (|eps h| / 2^5 < 0.1 for the spreads used here, so the truncation error
+    is below 1e-16).  Chunks of the stack are processed element-wise."""
     n = h.shape[0]
-    eye = np.broadcast_to(np.eye(3, dtype=np.complex128), (n, 3, 3))
-    result = eye.copy()
-    term = eye.copy()
-    for k in range(1, order + 1):
-        term = np.matmul(term, a) / k
-        result = result + term
-    for _ in range(squarings):
-        result = np.matmul(result, result)
-    return result
+    out = np.empty((n, 3, 3), dtype=np.complex128)
+    f = 1j * eps / (2 ** squarings)
+    for c0 in range(0, n, chunk):
+        hc = h[c0:c0 + chunk]
+        a = [[f * hc[:, i, j] for j in range(3)] for i in range(3)]
+        one = np.ones(hc.shape[0], dtype=np.complex128)
+        zero = np.zeros(hc.shape[0], dtype=np.complex128)
+        term = [[one if i == j else zero for j in range(3)] for i in range(3)]
+        res = [[term[i][j].copy() for j in range(3)] for i in range(3)]
+        for k in range(1, order + 1):
+            term = _mul3(term, a)
+            term = [[term[i][j] / k for j in range(3)] for i in range(3)]
+            res = [[res[i][j] + term[i][j] for j in range(3)] for i in range(3)]
+        for _ in range(squarings):
+            res = _mul3(res, res)
+        for i in range(3):
+            for j in range(3):
+                out[c0:c0 + chunk, i, j] = res[i][j]
+    return out
 
 
 def reunitarize(u):
@@ -47,16 +65,22 @@ def reunitarize(u):
     return np.stack([r0, r1, r2], axis=1)
 
 
-def random_su3(n, eps, seed, stream=0):
-    """n SU(3) matrices exp(i eps H), H traceless Hermitian with N(0,1) entries."""
+def random_su3(n, eps, seed, stream=0, chunk=1 << 16):
+    """n SU(3) matrices exp(i eps H), H traceless Hermitian with N(0,1) entries
+    (real parts of all n matrices drawn first, then the imaginary parts);
+    processed in chunks so the temporaries stay cache-sized."""
     g = _rng(seed, stream)
     re = g.standard_normal((n, 3, 3))
     im = g.standard_normal((n, 3, 3))
-    a = re + 1j * im
-    h = 0.5 * (a + np.conj(np.transpose(a, (0, 2, 1))))
-    tr = np.trace(h, axis1=1, axis2=2) / 3.0
-    h = h - tr[:, None, None] * np.eye(3)[None]
-    return reunitarize(_expm_i_eps_h(h, eps))
+    out = np.empty((n, 3, 3), dtype=np.complex128)
+    eye = np.eye(3)[None]
+    for c0 in range(0, n, chunk):
+        a = re[c0:c0 + chunk] + 1j * im[c0:c0 + chunk]
+        h = 0.5 * (a + np.conj(np.transpose(a, (0, 2, 1))))
+        tr = np.trace(h, axis1=1, axis2=2) / 3.0
+        h = h - tr[:, None, None] * eye
+        out[c0:c0 + chunk] = reunitarize(_expm_i_eps_h(h, eps))
+    return out
 
 
 def gauge_field(L, eps, seed):
