@@ -20,6 +20,12 @@ enum HopPrec {
 enum HopFlags {
   HOP_NT_STORE = 1,       // streaming (non-temporal) stores of the output
   HOP_NO_XCD_REMAP = 2,   // keep the hardware block order
+  HOP_SPATIAL = 4,        // XCD x owns spatial block x of the lattice (all rhs),
+                          // swept along t (Wilson kind, needs the XCD remap)
+  HOP_SITEWAVE = 8,       // a wavefront computes one site for 64/nb rhs (Wilson
+                          // kind, nb divides 64, no partials): the site's
+                          // neighbour entries and links are wave-uniform
+                          // (scalar loads), only the spinors are per lane
 };
 
 struct HopParams {
@@ -52,6 +58,10 @@ struct HopParams {
   double x5coef, x5mass;
   const uint4* nbr;         // neighbour table (k_build_nbr)
   const uint32_t* nbr_neg;
+  // HOP_SPATIAL: site_cb of logical site b*perm_S + q (the same table for
+  // both parities), perm_S = Vcb / 8 sites per spatial block
+  const uint32_t* perm;
+  int perm_S;
   int clov;
   int rhs_per_set;
   CloverRef clover;

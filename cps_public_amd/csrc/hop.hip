@@ -10,13 +10,16 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
   const int64_t nblk = (nelem + bs - 1) / bs;
   dim3 grid((unsigned)nblk), block(bs);
   const bool nt = (P.flags & HOP_NT_STORE) != 0;
-#define CPSHIP_HOP_LAUNCH(DAG, KIND)                                                                   \
-  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, true, CLOV_NONE>), grid, block, 0, st, P); \
-  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_NONE>), grid, block, 0, st, P);
+#define CPSHIP_HOP_LAUNCH(DAG, KIND, SW)                                                                      \
+  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, true, CLOV_NONE, SW>), grid, block, 0, st, P); \
+  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_NONE, SW>), grid, block, 0, st, P);
   if (P.kind == KIND_DWF) {
-    if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_DWF) } else { CPSHIP_HOP_LAUNCH(0, KIND_DWF) }
+    if (P.flags & HOP_SITEWAVE) return hipErrorInvalidValue;
+    if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_DWF, 0) } else { CPSHIP_HOP_LAUNCH(0, KIND_DWF, 0) }
+  } else if (P.flags & HOP_SITEWAVE) {
+    if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_WILSON, 1) } else { CPSHIP_HOP_LAUNCH(0, KIND_WILSON, 1) }
   } else {
-    if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_WILSON) } else { CPSHIP_HOP_LAUNCH(0, KIND_WILSON) }
+    if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_WILSON, 0) } else { CPSHIP_HOP_LAUNCH(0, KIND_WILSON, 0) }
   }
 #undef CPSHIP_HOP_LAUNCH
   return hipGetLastError();

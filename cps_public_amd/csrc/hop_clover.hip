@@ -19,9 +19,12 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
   const int64_t nblk = (nelem + bs - 1) / bs;
   dim3 grid((unsigned)nblk), block(bs);
   const bool nt = (P.flags & HOP_NT_STORE) != 0;
-#define CPSHIP_HOP_LAUNCH(DAG, CLOV)                                                                         \
-  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, true, CLOV>), grid, block, 0, st, P); \
-  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, false, CLOV>), grid, block, 0, st, P);
+#define CPSHIP_HOP_LAUNCH2(DAG, CLOV, SW)                                                                        \
+  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, true, CLOV, SW>), grid, block, 0, st, P);  \
+  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, false, CLOV, SW>), grid, block, 0, st, P);
+#define CPSHIP_HOP_LAUNCH(DAG, CLOV)                                   \
+  if (P.flags & HOP_SITEWAVE) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 1) }     \
+  else { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 0) }
   if (P.clov == CLOV_POST) {
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, CLOV_POST) } else { CPSHIP_HOP_LAUNCH(0, CLOV_POST) }
   } else if (P.clov == CLOV_X) {
@@ -32,6 +35,7 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
     return hipErrorInvalidValue;
   }
 #undef CPSHIP_HOP_LAUNCH
+#undef CPSHIP_HOP_LAUNCH2
   return hipGetLastError();
 }
 

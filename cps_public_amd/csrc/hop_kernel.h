@@ -488,6 +488,7 @@ __device__ __forceinline__ void block_partial(R v, double* partial, int64_t slot
     for (int k = 0; k < nw; k++) s += red[k];
     partial[slot] = s;
   }
+  __syncthreads();   // red[] is reused by the block's next sub-block
 }
 
 // Maps the hardware block id so that each XCD (blocks are dealt round-robin
@@ -517,16 +518,105 @@ template <> struct HopWaves<SpinD> { static constexpr int value = CPSHIP_HOP_WAV
 template <> struct HopWaves<SpinF> { static constexpr int value = CPSHIP_HOP_WAVES_F; };
 template <> struct HopWaves<SpinH> { static constexpr int value = CPSHIP_HOP_WAVES_H; };
 
-template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV>
-__global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) {
-  const int64_t nblk = (int64_t)gridDim.x;
-  const int64_t lb = logical_block(blockIdx.x, nblk, !(P.flags & HOP_NO_XCD_REMAP));
-  const int64_t e = lb * blockDim.x + threadIdx.x;   // element within this launch
+// Which output element thread threadIdx.x of logical sub-block `sb` (of
+// nsub) computes, and the partial-sum slot of that sub-block.
+struct HopElem {
+  int rhs, site, j;
+  int64_t pslot;   // rhs * blocks_per_rhs + sub-block within the rhs
+};
+
+// HOP_SITEWAVE decode (Wilson kind): sub-block sb = (spatial block b | -,
+// rhs group G, site block); wave w of the block computes site
+// (site block) * W + w for rhs G*g + lane / nb, g = 64 / nb.  The site is
+// wave-uniform.
+__device__ __forceinline__ HopElem hop_decode_sitewave(const HopParams& P, int64_t sb, int64_t nsub) {
+  HopElem E;
+  const int W = blockDim.x >> 6;
+  const int g = 64 / P.nb;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  E.rhs = 0;
+  int site;
+  if (P.flags & HOP_SPATIAL) {
+    const int64_t per_b = nsub >> 3;
+    const int b = (int)(sb / per_b);
+    const int64_t n_in = sb - (int64_t)b * per_b;
+    const int64_t spb = P.perm_S / W;   // site blocks per spatial block
+    const int G = (int)(n_in / spb);
+    const int64_t sblk = n_in - (int64_t)G * spb;
+    site = (int)P.perm[(int64_t)b * P.perm_S + sblk * W + w];
+    E.rhs = G * g + lane / P.nb;
+  } else {
+    const int64_t spb = (int64_t)P.nsites_out / W;
+    const int G = (int)(sb / spb);
+    const int64_t sblk = sb - (int64_t)G * spb;
+    site = (int)(sblk * W + w);
+    E.rhs = G * g + lane / P.nb;
+  }
+  E.site = __builtin_amdgcn_readfirstlane(site);
+  E.j = lane - (lane / P.nb) * P.nb;
+  E.pslot = 0;   // no partials in this mode
+  return E;
+}
+
+template <int KIND>
+__device__ __forceinline__ HopElem hop_decode(const HopParams& P, int64_t sb, int64_t nsub) {
+  HopElem E;
   const int64_t per_rhs = (int64_t)P.nsites_out * P.nb;
-  const int rhs = (int)(e / per_rhs);
-  const int64_t rem = e - (int64_t)rhs * per_rhs;
-  const int site = (int)(rem / P.nb);
-  const int j = (int)(rem - (int64_t)site * P.nb);
+  if (KIND == KIND_WILSON && (P.flags & HOP_SPATIAL)) {
+    // sb = (spatial block b = XCD, rhs, sub-block within (b, rhs)); all of it
+    // is uniform over the block (scalar arithmetic)
+    const int64_t per_b = nsub >> 3;
+    const int b = (int)(sb / per_b);
+    const int64_t n_in = sb - (int64_t)b * per_b;
+    const int64_t bpr = ((int64_t)P.perm_S * P.nb) / blockDim.x;   // sub-blocks per (b, rhs)
+    E.rhs = (int)(n_in / bpr);
+    const int64_t blk = n_in - (int64_t)E.rhs * bpr;
+    const int64_t el = blk * blockDim.x + threadIdx.x;
+    const int q = (int)(el / P.nb);
+    E.j = (int)(el - (int64_t)q * P.nb);
+    E.site = (int)P.perm[(int64_t)b * P.perm_S + q];
+    E.pslot = (int64_t)E.rhs * (bpr << 3) + (int64_t)b * bpr + blk;
+  } else {
+    const int64_t e = sb * blockDim.x + threadIdx.x;   // element within this launch
+    E.rhs = (int)(e / per_rhs);
+    const int64_t rem = e - (int64_t)E.rhs * per_rhs;
+    E.site = (int)(rem / P.nb);
+    E.j = (int)(rem - (int64_t)E.site * P.nb);
+    E.pslot = sb;
+  }
+  return E;
+}
+
+// neighbour indices and boundary bits of the element's site, precomputed per
+// (parity, site) by k_build_nbr (the j lanes of a site read one address)
+template <int KIND>
+__device__ __forceinline__ Nbr hop_nbr(const HopParams& P, const HopElem& E) {
+  const Geom& g = P.geom;
+  int out_par, site_cb;
+  if (KIND == KIND_DWF) {
+    out_par = E.site >= g.Vcb ? 1 : 0;
+    site_cb = E.site - out_par * g.Vcb;
+  } else {
+    out_par = P.out_parity;
+    site_cb = E.site;
+  }
+  Nbr N;
+  const int64_t ns = (int64_t)out_par * g.Vcb + site_cb;
+  const uint4 a = P.nbr[ns], b = P.nbr[2 * (int64_t)g.Vcb + ns];
+  N.idx[0] = (int)a.x; N.idx[1] = (int)a.y; N.idx[2] = (int)a.z; N.idx[3] = (int)a.w;
+  N.idx[4] = (int)b.x; N.idx[5] = (int)b.y; N.idx[6] = (int)b.z; N.idx[7] = (int)b.w;
+  N.neg = P.nbr_neg[ns];
+  return N;
+}
+
+// One output element: the hop, the fifth-dimension term (DWF), the epilogue
+// and the store; returns the element's |out|^2 when partials are requested.
+template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV>
+__device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, const Nbr& N) {
+  const int rhs = E.rhs, site = E.site, j = E.j;
+  const int64_t per_rhs = (int64_t)P.nsites_out * P.nb;
+  const int64_t rem = (int64_t)site * P.nb + j;
   const bool valid = rhs < P.nrhs;
   R nrm = R(0);
   if (valid) {
@@ -547,16 +637,6 @@ __global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) 
     cplx<R> acc[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) acc[k] = cmk<R>(R(0), R(0));
-    // neighbour indices and boundary bits of this site, precomputed per
-    // (parity, site) by k_build_nbr (the j lanes of a site read one address)
-    Nbr N;
-    {
-      const int64_t ns = (int64_t)out_par * g.Vcb + site_cb;
-      const uint4 a = P.nbr[ns], b = P.nbr[2 * (int64_t)g.Vcb + ns];
-      N.idx[0] = (int)a.x; N.idx[1] = (int)a.y; N.idx[2] = (int)a.z; N.idx[3] = (int)a.w;
-      N.idx[4] = (int)b.x; N.idx[5] = (int)b.y; N.idx[6] = (int)b.z; N.idx[7] = (int)b.w;
-      N.neg = P.nbr_neg[ns];
-    }
     constexpr int NB = HopBatch<SIN>::value;
     if constexpr (NB == 8) {
       hop_batch<0, 8, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
@@ -674,12 +754,21 @@ __global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) 
       for (int k = 0; k < 12; k++) nrm += res[k].re * res[k].re + res[k].im * res[k].im;
     }
   }
-  if (P.partial) {
-    // blocks never straddle two rhs (block size divides nsites_out*nb)
-    const int64_t bpr = per_rhs / blockDim.x;
-    block_partial<R>(nrm, P.partial, lb);
-    (void)bpr;
-  }
+  return nrm;
+}
+
+
+// SW = 0: one output element per thread in element order; SW = 1
+// (HOP_SITEWAVE): one site per wavefront, so the site's neighbour entries
+// and links are read with scalar loads.
+template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV, int SW>
+__global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) {
+  const int64_t nsub = (int64_t)gridDim.x;
+  const int64_t hb = logical_block(blockIdx.x, gridDim.x, !(P.flags & HOP_NO_XCD_REMAP));
+  const HopElem E = SW ? hop_decode_sitewave(P, hb, nsub) : hop_decode<KIND>(P, hb, nsub);
+  const Nbr N = hop_nbr<KIND>(P, E);
+  const R nrm = hop_element<SIN, SOUT, LK, R, DAG, KIND, NT, CLOV>(P, E, N);
+  if (P.partial) block_partial<R>(nrm, P.partial, E.pslot);
 }
 
 }  // namespace cpship

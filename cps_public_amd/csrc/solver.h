@@ -45,10 +45,13 @@ struct OpLayout {
 
 struct HopTuneKey {
   int kind, prec, dag, nb, nsites, nrhs, partial, hasX, clov;
+  int L[4];   // geometry: the cache is shared by every handle of the process
+  int mode;   // CPS_HIP_HOP_MODE restriction in force when tuned
   bool operator<(const HopTuneKey& o) const {
-    const int a[9] = {kind, prec, dag, nb, nsites, nrhs, partial, hasX, clov};
-    const int b[9] = {o.kind, o.prec, o.dag, o.nb, o.nsites, o.nrhs, o.partial, o.hasX, o.clov};
-    for (int k = 0; k < 9; k++)
+    const int a[14] = {kind, prec, dag, nb, nsites, nrhs, partial, hasX, clov, L[0], L[1], L[2], L[3], mode};
+    const int b[14] = {o.kind, o.prec, o.dag, o.nb, o.nsites, o.nrhs, o.partial, o.hasX, o.clov,
+                       o.L[0], o.L[1], o.L[2], o.L[3], o.mode};
+    for (int k = 0; k < 14; k++)
       if (a[k] != b[k]) return a[k] < b[k];
     return false;
   }
@@ -60,7 +63,6 @@ struct HopTuneVal {
 
 struct Lattice {
   Geom g;
-  std::map<HopTuneKey, HopTuneVal> hop_tune;
   int ls = 1;
   hipStream_t stream = nullptr;
   DevBuf gauge_aos;
@@ -71,6 +73,11 @@ struct Lattice {
   std::vector<Field> scratch;
   DevBuf partial, scal, io, io2;
   DevBuf nbr, nbr_neg;   // neighbour table of the geometry (hop kernels)
+  // spatial site order of the HOP_SPATIAL launches (built on first use):
+  // 0 not built, 1 built, -1 the geometry has no 8-way (y, z) split
+  DevBuf perm;
+  int perm_state = 0;
+  int ensure_perm();
   // clover sets [kappa][cb] (cb 0: A_ee^-1, cb 1: A_oo) per storage format;
   // FP64 is built from the gauge, FP32/FP16 are converted from it on demand
   DevBuf clover[NCLOVER];

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace cpship {
@@ -140,6 +141,45 @@ GaugeRef Lattice::gauge_ref(int link_storage) const {
   return gr;
 }
 
+// The (y, z) plane is split into 8 blocks (by x bz = 8, the split with the
+// smallest halo), one per XCD; inside a block the order is t, z, y, x, so an
+// XCD sweeps its block along t and the t-neighbour reuse stays within the
+// window of a few block slices its L2 holds.
+int Lattice::ensure_perm() {
+  if (perm_state) return perm_state > 0 ? 0 : CPS_HIP_ERR_ARG;
+  int best_by = 0, best_bz = 0;
+  double best = 1e30;
+  for (int by = 1; by <= 8; by *= 2) {
+    const int bz = 8 / by;
+    if (g.L[1] % by || g.L[2] % bz) continue;
+    const double ny = g.L[1] / by, nz = g.L[2] / bz;
+    const double halo = (ny + 2) * (nz + 2) / (ny * nz);
+    if (halo < best) {
+      best = halo;
+      best_by = by;
+      best_bz = bz;
+    }
+  }
+  if (!best_by) {
+    perm_state = -1;
+    return CPS_HIP_ERR_ARG;
+  }
+  const int lh = g.L[0] / 2, ny = g.L[1] / best_by, nz = g.L[2] / best_bz;
+  std::vector<uint32_t> h((size_t)g.Vcb);
+  size_t q = 0;
+  for (int bz = 0; bz < best_bz; bz++)
+    for (int by = 0; by < best_by; by++)
+      for (int t = 0; t < g.L[3]; t++)
+        for (int z = bz * nz; z < (bz + 1) * nz; z++)
+          for (int y = by * ny; y < (by + 1) * ny; y++)
+            for (int xh = 0; xh < lh; xh++) h[q++] = (uint32_t)(xh + lh * (y + g.L[1] * (z + g.L[2] * t)));
+  int rc = perm.ensure(h.size() * sizeof(uint32_t));
+  if (rc) return rc;
+  CPSHIP_CHECK(hipMemcpy(perm.p, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  perm_state = 1;
+  return 0;
+}
+
 Field& Lattice::field(int slot, int storage, int64_t nelem) {
   Field& f = scratch[slot];   // slot < S_NSLOTS by construction
   if (f.storage != storage || f.nelem != nelem || !f.buf.p) {
@@ -260,16 +300,55 @@ static bool tuning_enabled() {
   return v == 1;
 }
 
+// whether a launch may use HOP_SPATIAL at block size b
+static bool spatial_ok(Lattice& L, const HopParams& P, int b) {
+  if (P.kind != KIND_WILSON || L.g.Vcb % 8) return false;
+  const long long per_b = (long long)(L.g.Vcb / 8) * P.nb;
+  if (per_b % b) return false;
+  return L.ensure_perm() == 0;
+}
+
+// whether a launch may use HOP_SITEWAVE at block size b (one site per
+// wavefront for 64/nb rhs; launches with partials keep the element order)
+static bool sitewave_ok(Lattice& L, const HopParams& P, int b, bool spatial) {
+  if (P.kind != KIND_WILSON || P.partial || b < 64 || 64 % P.nb) return false;
+  const int W = b / 64, g = 64 / P.nb;
+  if (P.nrhs % g || P.nsites_out % W) return false;
+  if (spatial && (L.g.Vcb / 8) % W) return false;
+  return true;
+}
+
 int tuned_launch(Lattice& L, HopParams& P) {
   P.nbr = (const uint4*)L.nbr.p;
   P.nbr_neg = (const uint32_t*)L.nbr_neg.p;
+  P.perm = nullptr;
+  P.perm_S = L.g.Vcb / 8;
   const long long per_rhs = (long long)P.nsites_out * P.nb;
   const int def_block = pick_block(per_rhs);
+  if (P.kind == KIND_WILSON && L.perm_state >= 0 && L.g.Vcb % 8 == 0 && L.ensure_perm() == 0)
+    P.perm = (const uint32_t*)L.perm.p;
+  int mode = 0;
+  if (const char* m = getenv("CPS_HIP_HOP_MODE")) mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : 0;
   HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr,
-                 P.clov};
-  auto it = L.hop_tune.find(key);
-  if (it == L.hop_tune.end()) {
-    HopTuneVal best{def_block, 0, 0.0};
+                 P.clov, {L.g.L[0], L.g.L[1], L.g.L[2], L.g.L[3]}, mode};
+  // the winners are kept per process (every handle, engine and thread of the
+  // same geometry reuses them); a launch shape is timed once
+  static std::mutex tune_mu;
+  static std::map<HopTuneKey, HopTuneVal> tune_cache;
+  HopTuneVal chosen;
+  bool have = false;
+  {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    auto f = tune_cache.find(key);
+    if (f != tune_cache.end()) {
+      chosen = f->second;
+      have = true;
+    }
+  }
+  if (!have) {
+    // launches that write reduction partials use a fixed order (results do
+    // not depend on tuning): spatial whenever the geometry allows it
+    HopTuneVal best{def_block, spatial_ok(L, P, def_block) ? HOP_SPATIAL : 0, 0.0};
     if (tuning_enabled()) {
       std::vector<int> blocks;
       if (P.partial) blocks.push_back(def_block);
@@ -281,7 +360,15 @@ int tuned_launch(Lattice& L, HopParams& P) {
       CPSHIP_CHECK(hipEventCreate(&e1));
       float best_ms = 1e30f;
       for (int b : blocks)
-        for (int f = 0; f < 4; f++) {
+        for (int f = 0; f < 16; f++) {
+          if (f & HOP_SPATIAL) {
+            if ((f & HOP_NO_XCD_REMAP) || !spatial_ok(L, P, b)) continue;
+          }
+          if ((f & HOP_SITEWAVE) && !sitewave_ok(L, P, b, (f & HOP_SPATIAL) != 0)) continue;
+          // CPS_HIP_HOP_MODE=sitewave / element restricts the candidates
+          // (where site-per-wave is possible) to that thread mapping
+          if (mode && sitewave_ok(L, P, b, false) && ((f & HOP_SITEWAVE) != 0) != (mode == 2)) continue;
+          if (P.partial && (f & HOP_SPATIAL) != (best.flags & HOP_SPATIAL)) continue;
           P.block = b;
           P.flags = f;
           CPSHIP_CHECK(launch_hop(P, L.stream));   // warm-up
@@ -300,11 +387,20 @@ int tuned_launch(Lattice& L, HopParams& P) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       best.ms = best_ms / 3;
+      static const int report = [] {
+        const char* e = getenv("CPS_HIP_TUNE_REPORT");
+        return e && *e == '1';
+      }();
+      if (report)
+        fprintf(stderr, "cps_hip tune: kind %d prec %d dag %d nb %d nsites %d nrhs %d clov %d -> block %d flags %d (%.3f ms)\n",
+                P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.clov, best.block, best.flags, best.ms);
     }
-    it = L.hop_tune.emplace(key, best).first;
+    chosen = best;
+    std::lock_guard<std::mutex> lk(tune_mu);
+    tune_cache.emplace(key, best);
   }
-  P.block = it->second.block;
-  P.flags = it->second.flags;
+  P.block = chosen.block;
+  P.flags = chosen.flags;
   CPSHIP_CHECK(launch_hop(P, L.stream));
   return 0;
 }

@@ -615,3 +615,37 @@ def test_qpropw_point_propagator(kind, x0, prec):
     print("qpropw", kind, x0, prec, [r.iterations for r in res], worst, flush=True)
     assert max(r.true_res for r in res) <= 1e-10
     assert worst < (1e-9 if prec == C.PREC_DOUBLE else 1e-8)
+
+
+# ---------------------------------------------------------------------------
+# site-per-wavefront thread mapping of the hop (HOP_SITEWAVE)
+
+
+@pytest.mark.parametrize("prec", [C.PREC_DOUBLE, C.PREC_MIXED_HALF])
+def test_sitewave_mapping_bit_identical(oracle, monkeypatch, prec):
+    """The Moebius operator with Ls = 16 and 4 rhs (one site per wavefront)
+    gives the same bits as the element mapping, and matches the oracle."""
+    L, ls, nrhs = [4, 4, 4, 8], 16, 4
+    mass, m5, b, c = 0.05, 1.8, 1.5, 0.5
+    u = wilson_gauge(L, 0.2, 210)
+    op = C.make_op(C.OP_MOBIUS, mass=mass, m5=m5, b=b, c=c)
+    vcb = int(np.prod(L)) // 2
+    src = np.concatenate([F.gaussian_fermion(vcb * ls, 211 + k) for k in range(nrhs)])
+    res = {}
+    for mode in ("element", "sitewave"):
+        monkeypatch.setenv("CPS_HIP_HOP_MODE", mode)
+        lat = C.Lattice(L, ls, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        if prec == C.PREC_DOUBLE:
+            res[mode] = (lat.matpc(op, src, 0, nrhs=nrhs), lat.matpc(op, src, 1, nrhs=nrhs))
+        else:
+            res[mode] = lat.mdagm_prec(op, src, prec, nrhs=nrhs)
+        lat.close()
+    for a, b_ in zip(res["element"], res["sitewave"]):
+        assert np.array_equal(a, b_)
+    if prec == C.PREC_DOUBLE:
+        m = oracle.mobius_struct(L, ls, m5, b, c)
+        n = 24 * vcb * ls
+        for k in range(nrhs):
+            ref = oracle.mobius_m(u, src[k * n:(k + 1) * n], mass, m)
+            assert rel(res["sitewave"][0][k * n:(k + 1) * n], ref) < 1e-13
