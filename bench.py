@@ -260,6 +260,8 @@ def main():
     ap.add_argument("--hop-reps", type=int, default=50)
     ap.add_argument("--max-iter", type=int, default=20000,
                     help="CG iteration cap (a small cap gives a bounded run for kernel traces)")
+    ap.add_argument("--reliable-delta", type=float, default=0.1,
+                    help="mixed CG: reliable update when |r_sloppy| < delta |r| of the last update")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the hopping kernels (mixed and FP64) and the reference copy kernel, "
                          "for hardware-counter runs")
@@ -294,7 +296,8 @@ def main():
         lat.close()
         return
     src_dev = C.DeviceBuffer.from_array(src)
-    arg_main = C.make_cg_arg(1e-10, a.max_iter, precision=prec_main, reliable_every=100)
+    arg_main = C.make_cg_arg(1e-10, a.max_iter, precision=prec_main, reliable_every=100,
+                             reliable_delta=a.reliable_delta)
     t_main, res_main = time_solves(lat, op, src_dev, arg_main, nrhs, a.warmup, a.steps)
     ms = 1e3 * float(np.mean(t_main))
     s_per_rhs = float(np.mean(t_main)) / nrhs
