@@ -90,6 +90,8 @@ def _declare(l):
         "cps_hip_invcg_device": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_matinv": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_qpropw_point": [vp, P(Op), P(ctypes.c_int), P(CgArg), vp, P(CgResult)],
+        "cps_hip_gauge_import_canonical": [vp, vp],
+        "cps_hip_fmatinv": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_bench_hop": [vp, i, i, i, i, P(d), P(d)],
         "cps_hip_bench_copy": [vp, ctypes.c_longlong, i, P(d)],
         "cps_hip_device_count": [],
@@ -256,6 +258,21 @@ class Lattice:
         res = (CgResult * nrhs)()
         _check(lib().cps_hip_matinv(self.h, ctypes.byref(op), _ptr(out), _ptr(b), nrhs, ctypes.byref(arg), res),
                "cps_hip_matinv")
+        return out, [res[k] for k in range(nrhs)]
+
+    def import_gauge_canonical(self, u_canon):
+        """CANONICAL gauge without boundary signs; the handle's bc signs are
+        applied on the device (Lattice::Convert + BondCond)."""
+        a = np.ascontiguousarray(u_canon, dtype=np.float64)
+        _check(lib().cps_hip_gauge_import_canonical(self.h, _ptr(a)), "cps_hip_gauge_import_canonical")
+
+    def fmatinv(self, op, b, arg, guess=None, nrhs=1):
+        """Lattice::FmatInv on CANONICAL full vectors (5-D ones s-major)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.zeros_like(b) if guess is None else np.ascontiguousarray(guess, dtype=np.float64).copy()
+        res = (CgResult * nrhs)()
+        _check(lib().cps_hip_fmatinv(self.h, ctypes.byref(op), _ptr(out), _ptr(b), nrhs, ctypes.byref(arg), res),
+               "cps_hip_fmatinv")
         return out, [res[k] for k in range(nrhs)]
 
     def qpropw_point(self, op, x0, arg):

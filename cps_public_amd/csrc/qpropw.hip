@@ -74,27 +74,26 @@ int matinv_device(Lattice& L, const cps_hip_op* op, Field& in_o, Field& in_e, Fi
   return 0;
 }
 
-namespace {
-
-// Where a 4-D site's spin-colour component lives in the device fields of
-// an operator: the checkerboard (odd or even field) and the element index
-// within one rhs.  For the 5-D operators s is the slice; DWF fields hold
-// slice s = 2j + ((p + cb5) & 1) at element (p Vcb + site_cb) Ls/2 + j,
-// Moebius fields Ls slots per site of one 4-D parity.
-template <int KIND>
-__host__ __device__ __forceinline__ void field_slot(int p, int site_cb, int s, int Vcb, int ls, bool* odd,
-                                                    int64_t* e) {
-  if constexpr (KIND == CPS_HIP_OP_DWF) {
-    *odd = ((p + s) & 1) != 0;
-    *e = ((int64_t)p * Vcb + site_cb) * (ls / 2) + (s >> 1);
-  } else if constexpr (KIND == CPS_HIP_OP_MOBIUS) {
-    *odd = p != 0;
-    *e = (int64_t)site_cb * ls + s;
-  } else {
-    *odd = p != 0;
-    *e = site_cb;
-  }
+// Fmobius::FmatInv multiplies the source by D_minus first (dminus = 1 by
+// default, lattice.h:1124; f_mobius.C:68-78): DiracOpMobius::Dminus
+// (d_op_mobius.C:745-778) = -(c D_W - 1/a5) with D_W = (4 - M5) - H/2, i.e.
+// (1/a5 - c (4 - M5)) in + (c/2) H in on each checkerboard; applied in place
+// to the odd / even source fields.
+int mobius_dminus_fields(Lattice& L, const cps_hip_op* op, Field& f_odd, Field& f_even, int nrhs) {
+  const int64_t n = (int64_t)L.g.Vcb * L.ls * nrhs;
+  Field& t3 = L.field(S_T3, STORE_D, n);
+  Field& tmp = L.field(S_TMP, STORE_D, n);
+  if (L.last_rc) return L.last_rc;
+  const double xc = op->a5_inv - op->mobius_c * (4 - op->m5), hc = 0.5 * op->mobius_c;
+  int rc;
+  if ((rc = hop(L, op, PREC_D_D18, t3.ref(), f_even.ref(), 1, 0, hc, f_odd.ref(), xc, nullptr, nrhs))) return rc;
+  if ((rc = hop(L, op, PREC_D_D18, tmp.ref(), f_odd.ref(), 0, 0, hc, f_even.ref(), xc, nullptr, nrhs))) return rc;
+  CPSHIP_CHECK(convert_spinor(STORE_D, t3.ref(), STORE_D, f_odd.ref(), n, L.stream));
+  CPSHIP_CHECK(convert_spinor(STORE_D, tmp.ref(), STORE_D, f_even.ref(), n, L.stream));
+  return 0;
 }
+
+namespace {
 
 // The 12 source entries (one double2 each), addresses computed on the host.
 struct PointTargets {
@@ -203,24 +202,8 @@ extern "C" int cps_hip_qpropw_point(cps_hip_lattice* lat, const cps_hip_op* op, 
   }
   hipLaunchKernelGGL(k_point_src, dim3(1), dim3(64), 0, L.stream, T);
   CPSHIP_CHECK(hipGetLastError());
-  Field* so = &fo;
-  Field* se = &fe;
-  if (op->kind == CPS_HIP_OP_MOBIUS) {
-    // Fmobius::FmatInv multiplies the source by D_minus first (dminus = 1 by
-    // default, lattice.h:1124; f_mobius.C:68-78): DiracOpMobius::Dminus
-    // (d_op_mobius.C:745-778) = -(c D_W - 1/a5) with D_W = (4 - M5) - H/2,
-    // i.e. (1/a5 - c (4 - M5)) in + (c/2) H in on each checkerboard.
-    Field& t3 = L.field(S_T3, STORE_D, n);
-    Field& tmp = L.field(S_TMP, STORE_D, n);
-    if (L.last_rc) return L.last_rc;
-    const double xc = op->a5_inv - op->mobius_c * (4 - op->m5), hc = 0.5 * op->mobius_c;
-    if ((rc = hop(L, op, PREC_D_D18, t3.ref(), fe.ref(), 1, 0, hc, fo.ref(), xc, nullptr, nrhs))) return rc;
-    if ((rc = hop(L, op, PREC_D_D18, tmp.ref(), fo.ref(), 0, 0, hc, fe.ref(), xc, nullptr, nrhs))) return rc;
-    // the prepared source moves into the io slots (t3, tmp are MatInv scratch)
-    CPSHIP_CHECK(convert_spinor(STORE_D, t3.ref(), STORE_D, fo.ref(), n, L.stream));
-    CPSHIP_CHECK(convert_spinor(STORE_D, tmp.ref(), STORE_D, fe.ref(), n, L.stream));
-  }
-  if ((rc = matinv_device(L, op, *so, *se, x, eo, nrhs, arg, res))) return rc;
+  if (op->kind == CPS_HIP_OP_MOBIUS && (rc = mobius_dminus_fields(L, op, fo, fe, nrhs))) return rc;
+  if ((rc = matinv_device(L, op, fo, fe, x, eo, nrhs, arg, res))) return rc;
   // Moebius: the preconditioned system is 2 kappa_b times the operator
   // (d_op_mobius.C:628-633)
   const double fac =

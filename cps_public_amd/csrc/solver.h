@@ -94,6 +94,7 @@ struct Lattice {
   int import_gauge(const double* u_host);
   int import_gauge_device(const double* u_dev);   // same, from device memory
   int import_gauge_from(const double* u, hipMemcpyKind kind);
+  int import_gauge_canonical(const double* u_canon);   // CANONICAL order + BondCond (canonical.hip)
   int ensure_gauge(int link_storage);
   GaugeRef gauge_ref(int link_storage) const;
   Field& field(int slot, int storage, int64_t nelem);
@@ -203,12 +204,40 @@ int clover_matinv_reconstruct(Lattice& L, const cps_hip_op* op, const FieldRef& 
                               const FieldRef& in_e, const FieldRef& t1, const FieldRef& t2, const FieldRef& t3,
                               int nrhs);
 
+// Where a 4-D site's spin-colour component lives in the device fields of
+// an operator: the checkerboard (odd or even field) and the element index
+// within one rhs.  For the 5-D operators s is the slice; DWF fields hold
+// slice s = 2j + ((p + cb5) & 1) at element (p Vcb + site_cb) Ls/2 + j,
+// Moebius fields Ls slots per site of one 4-D parity.
+template <int KIND>
+__host__ __device__ __forceinline__ void field_slot(int p, int site_cb, int s, int Vcb, int ls, bool* odd,
+                                                    int64_t* e) {
+  if constexpr (KIND == CPS_HIP_OP_DWF) {
+    *odd = ((p + s) & 1) != 0;
+    *e = ((int64_t)p * Vcb + site_cb) * (ls / 2) + (s >> 1);
+  } else if constexpr (KIND == CPS_HIP_OP_MOBIUS) {
+    *odd = p != 0;
+    *e = (int64_t)site_cb * ls + s;
+  } else {
+    *odd = p != 0;
+    *e = site_cb;
+  }
+}
+
 // MatInv on device fields: in_o / in_e the source checkerboards (odd =
 // the CG's checkerboard), x the initial guess and on return the odd
 // solution, out_e the even solution; Moebius results are 2 kappa_b times the
 // DiracOpMobius::MatInv ones (the caller scales)
 int matinv_device(Lattice& L, const cps_hip_op* op, Field& in_o, Field& in_e, Field& x, Field& out_e, int nrhs,
                   const cps_hip_cg_arg* arg, cps_hip_cg_result* res);
+
+// Moebius D_minus applied in place to the source checkerboards (FmatInv)
+int mobius_dminus_fields(Lattice& L, const cps_hip_op* op, Field& f_odd, Field& f_even, int nrhs);
+// CANONICAL <-> device fields (canonical.hip): rhs-major canonical vectors,
+// [rhs][s][t][z][y][x][spin][colour][re,im]; scale multiplies the values
+int pack_canonical(Lattice& L, int kind, const double* canon, Field& f_odd, Field& f_even, int nrhs, double scale);
+int unpack_canonical(Lattice& L, int kind, const Field& f_odd, const Field& f_even, double* canon, int nrhs,
+                     double scale);
 
 // host <-> device field transfer with layout conversion; with on_device the
 // double pointer is device memory in the CPS layout (no PCIe copy)

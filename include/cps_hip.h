@@ -176,6 +176,25 @@ int cps_hip_invcg_device(cps_hip_lattice *lat, const cps_hip_op *op, double *sol
 int cps_hip_matinv(cps_hip_lattice *lat, const cps_hip_op *op, double *out, const double *in, int nrhs,
                    const cps_hip_cg_arg *arg, cps_hip_cg_result *res);
 
+/* ---- CANONICAL order (Lattice::Convert / Fconvert on the device) ----------
+ * cps_hip_gauge_import_canonical: u is the CANONICAL gauge field
+ * [t][z][y][x][mu][row][col][re,im] (lattice_base.C:5350-5351) WITHOUT
+ * boundary signs; the import transposes it to the WILSON order
+ * (convert_func.C:318-415) and applies the signs of the handle's bc
+ * (Lattice::BondCond, lattice_base.C:5313-5369), replacing what the DiracOp
+ * constructor does to the Lattice's field (dirac_op_base.C:164,
+ * d_op_wilson.C:118-121). */
+int cps_hip_gauge_import_canonical(cps_hip_lattice *lat, const double *u_canon);
+/* cps_hip_fmatinv: Lattice::FmatInv with CNV_FRM_YES (Fwilson::FmatInv
+ * f_wilson.C:269-285, FdwfBase / Fmobius::FmatInv f_mobius.C:45-110,
+ * Fclover::FmatInv): in and out are CANONICAL full vectors, 5-D ones s-major
+ * ([s][t][z][y][x][spin][colour][re,im]), nrhs of them back to back; out
+ * holds the initial guess.  The conversion to and from the DiracOp order
+ * runs on the device.  Moebius multiplies the source by D_minus first, as
+ * Fmobius::FmatInv does by default (dminus = 1, lattice.h:1124). */
+int cps_hip_fmatinv(cps_hip_lattice *lat, const cps_hip_op *op, double *out, const double *in, int nrhs,
+                    const cps_hip_cg_arg *arg, cps_hip_cg_result *res);
+
 /* ---- point-source propagator (QPropW::Run, QPropW.C:566-613) -------------
  * The 12 spin-colour point sources at site x0 (FermionVectorTp::
  * SetPointSource, FermionVector.C:1567-1644; for DWF/Moebius placed at

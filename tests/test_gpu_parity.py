@@ -649,3 +649,77 @@ def test_sitewave_mapping_bit_identical(oracle, monkeypatch, prec):
         for k in range(nrhs):
             ref = oracle.mobius_m(u, src[k * n:(k + 1) * n], mass, m)
             assert rel(res["sitewave"][0][k * n:(k + 1) * n], ref) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# CANONICAL order on the device: gauge import (Convert + BondCond) and
+# Lattice::FmatInv on CANONICAL vectors
+
+
+def test_gauge_import_canonical_equals_wilson_import():
+    """Importing the CANONICAL field (signs applied on the device) gives the
+    same operator, bit for bit, as importing the WILSON-order field with the
+    signs applied on the host."""
+    L, ls = [4, 4, 4, 8], 4
+    u = F.gauge_field(L, 0.2, 220)
+    src = F.gaussian_fermion(int(np.prod(L)) // 2 * ls, 221)
+    op = C.make_op(C.OP_DWF, mass=0.05, m5=1.8)
+    outs = []
+    for canon in (False, True):
+        lat = C.Lattice(L, ls, (0, 0, 0, 1))
+        if canon:
+            lat.import_gauge_canonical(u)
+        else:
+            lat.import_gauge(wilson_gauge(L, 0.2, 220))
+        outs.append(lat.matpc(op, src, 0))
+        lat.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("kind,prec", [("wilson", C.PREC_DOUBLE), ("clover", C.PREC_DOUBLE), ("dwf", C.PREC_DOUBLE),
+                                       ("mobius", C.PREC_DOUBLE), ("mobius", C.PREC_MIXED_HALF)])
+def test_fmatinv_canonical(kind, prec):
+    """Lattice::FmatInv on CANONICAL vectors (two rhs): D out = in for the full
+    textbook operator (Moebius: D out = D_minus in, the default dminus)."""
+    import scipy.sparse as sp
+    from tests import textbook as T
+    from cps_public_amd import layout as LY
+    L = [4, 4, 4, 4]
+    V = int(np.prod(L))
+    ls = 1 if kind in ("wilson", "clover") else 4
+    u = F.gauge_field(L, 0.2, 230)
+    H = T.hopping(T.links(u, L), L, antiperiodic=(0, 0, 0, 1))
+    mass, m5, b, c = 0.1, 1.8, 1.5, 0.5
+    lat = C.Lattice(L, ls, (0, 0, 0, 1))
+    lat.import_gauge_canonical(u)
+    lhs = None
+    if kind == "wilson":
+        op = C.make_op(C.OP_WILSON, mass=mass)
+        D = sp.identity(12 * V) - (1.0 / (2 * (mass + 4))) * H
+    elif kind == "clover":
+        kappa, csw = 0.12, 1.1
+        lat.clover_build([kappa], csw)
+        op = C.make_op(C.OP_CLOVER, mass=1.0 / (2 * kappa) - 4.0, csw=csw)
+        D = T.block_diag_sparse(T.clover_blocks(T.links(LY.bond_cond(u, L, [0, 0, 0, 1]), L), L, kappa, csw)) - kappa * H
+    elif kind == "dwf":
+        op = C.make_op(C.OP_DWF, mass=mass, m5=m5)
+        D = T.dwf_operator(H, ls, mass, m5)[0]
+    else:
+        op = C.make_op(C.OP_MOBIUS, mass=mass, m5=m5, b=b, c=c)
+        Dm, _ = T.mobius_operator(H, ls, mass, m5, b, c)
+        kb, _ = T.mobius_kappas(m5, b, c)
+        D = Dm / (2 * kb)
+        lhs = (1 - c * (4 - m5)) * sp.identity(12 * V * ls) + 0.5 * c * sp.kron(sp.identity(ls), H)
+    rng = np.random.default_rng(231)
+    nrhs = 2
+    bz = rng.standard_normal((nrhs, 12 * V * ls)) + 1j * rng.standard_normal((nrhs, 12 * V * ls))
+    breal = np.stack([bz.real, bz.imag], axis=-1).reshape(-1)
+    out, res = lat.fmatinv(op, breal, C.make_cg_arg(1e-12, 5000, precision=prec), nrhs=nrhs)
+    lat.close()
+    o = out.reshape(nrhs, -1, 2)
+    xz = o[..., 0] + 1j * o[..., 1]
+    for k in range(nrhs):
+        rhs = bz[k] if lhs is None else lhs @ bz[k]
+        r = D @ xz[k] - rhs
+        print("fmatinv", kind, prec, res[k].iterations, np.linalg.norm(r) / np.linalg.norm(rhs), flush=True)
+        assert np.linalg.norm(r) / np.linalg.norm(rhs) < 1e-9
