@@ -20,7 +20,9 @@ HEADER_PATH = os.path.join(_HERE, "..", "include", "cps_hip.h")
 OP_WILSON, OP_DWF, OP_MOBIUS, OP_CLOVER = 0, 1, 2, 3
 PREC_DOUBLE, PREC_MIXED_HALF, PREC_MIXED_BF16, PREC_MIXED_SINGLE = 0, 1, 2, 3
 ERR_NAMES = {-1: "bad argument", -2: "HIP runtime failure", -3: "no gauge field resident",
-             -4: "CG d == 0 (inv_cg.C:326-329)", -5: "device allocation failed", -6: "no HIP device"}
+             -4: "CG d == 0 (inv_cg.C:326-329)", -5: "device allocation failed", -6: "no HIP device",
+             -7: "file missing, truncated or malformed", -8: "NERSC checksum mismatch",
+             -9: "NERSC plaquette mismatch"}
 
 
 class CpsHipError(RuntimeError):
@@ -45,6 +47,13 @@ class ServiceStats(ctypes.Structure):
     _fields_ = [("requests_done", ctypes.c_longlong), ("rhs_done", ctypes.c_longlong),
                 ("iterations", ctypes.c_longlong), ("peak_bytes", ctypes.c_longlong),
                 ("engines_created", ctypes.c_int), ("engines_live", ctypes.c_int), ("last_error", ctypes.c_int)]
+
+
+class NerscInfo(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int * 4), ("recon_row_3", ctypes.c_int), ("bytes_per_real", ctypes.c_int),
+                ("big_endian", ctypes.c_int), ("checksum_header", ctypes.c_uint), ("checksum_data", ctypes.c_uint),
+                ("plaquette_header", ctypes.c_double), ("plaquette_data", ctypes.c_double),
+                ("link_trace_header", ctypes.c_double), ("link_trace_data", ctypes.c_double)]
 
 
 class CgResult(ctypes.Structure):
@@ -91,6 +100,9 @@ def _declare(l):
         "cps_hip_matinv": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_qpropw_point": [vp, P(Op), P(ctypes.c_int), P(CgArg), vp, P(CgResult)],
         "cps_hip_gauge_import_canonical": [vp, vp],
+        "cps_hip_nersc_read_header": [ctypes.c_char_p, P(NerscInfo)],
+        "cps_hip_nersc_read": [ctypes.c_char_p, vp, P(NerscInfo), ctypes.c_double],
+        "cps_hip_nersc_write": [ctypes.c_char_p, P(ctypes.c_int), vp, i, i, ctypes.c_char_p, i],
         "cps_hip_fmatinv": [vp, P(Op), vp, vp, i, P(CgArg), P(CgResult)],
         "cps_hip_bench_hop": [vp, i, i, i, i, P(d), P(d)],
         "cps_hip_bench_copy": [vp, ctypes.c_longlong, i, P(d)],
@@ -131,6 +143,27 @@ def make_op(kind, mass=0.0, m5=1.8, a5_inv=1.0, b=1.0, c=0.0, csw=0.0):
 def make_cg_arg(stop_rsd=1e-10, max_num_iter=10000, precision=PREC_DOUBLE, reliable_every=100,
                 reliable_delta=0.1, check_every=0, use_graph=1):
     return CgArg(max_num_iter, stop_rsd, precision, reliable_every, reliable_delta, check_every, use_graph)
+
+
+def nersc_read(path, chkprec=0.01):
+    """ReadLatticeParallel: the CANONICAL FP64 gauge field of a NERSC file and
+    its header/check information (raises CpsHipError on a checksum or
+    plaquette mismatch)."""
+    info = NerscInfo()
+    _check(lib().cps_hip_nersc_read_header(path.encode(), ctypes.byref(info)), "cps_hip_nersc_read_header")
+    u = np.zeros(int(np.prod(list(info.dims))) * 72)
+    rc = lib().cps_hip_nersc_read(path.encode(), _ptr(u), ctypes.byref(info), float(chkprec))
+    _check(rc, "cps_hip_nersc_read")
+    return u, info
+
+
+def nersc_write(path, dims, u_canon, recon_row_3=1, bytes_per_real=8, ensemble_id="", sequence_number=0):
+    """WriteLatticeParallel: a CANONICAL FP64 gauge field to a NERSC file
+    (IEEE32BIG or IEEE64BIG)."""
+    d = (ctypes.c_int * 4)(*[int(v) for v in dims])
+    u = np.ascontiguousarray(u_canon, dtype=np.float64)
+    _check(lib().cps_hip_nersc_write(path.encode(), d, _ptr(u), int(recon_row_3), int(bytes_per_real),
+                                     ensemble_id.encode(), int(sequence_number)), "cps_hip_nersc_write")
 
 
 class Lattice:

@@ -40,6 +40,9 @@ extern "C" {
 #define CPS_HIP_ERR_D_ZERO (-4)    /* CG: <p,Ap> = 0 (inv_cg.C:326-329) */
 #define CPS_HIP_ERR_NOMEM (-5)     /* device allocation failed / over budget */
 #define CPS_HIP_ERR_NODEV (-6)     /* no HIP device */
+#define CPS_HIP_ERR_IO (-7)        /* file missing, truncated or malformed */
+#define CPS_HIP_ERR_CHECKSUM (-8)  /* NERSC checksum mismatch (ReadLatticePar.C:161-176) */
+#define CPS_HIP_ERR_PLAQUETTE (-9) /* NERSC plaquette mismatch (ReadLatticePar.C:283-286) */
 
 /* operator kinds */
 #define CPS_HIP_OP_WILSON 0
@@ -194,6 +197,32 @@ int cps_hip_gauge_import_canonical(cps_hip_lattice *lat, const double *u_canon);
  * Fmobius::FmatInv does by default (dminus = 1, lattice.h:1124). */
 int cps_hip_fmatinv(cps_hip_lattice *lat, const cps_hip_op *op, double *out, const double *in, int nrhs,
                     const cps_hip_cg_arg *arg, cps_hip_cg_result *res);
+
+/* ---- NERSC gauge files (ReadLatticeParallel, ReadLatticePar.C:60-286;
+ * LatticeHeader::read/write, latheader.C:210-312) -------------------------
+ * Host-side reader and writer of the NERSC format: DATATYPE 4D_SU3_GAUGE
+ * (two rows stored, third rebuilt) or 4D_SU3_GAUGE_3x3, FLOATING_POINT
+ * IEEE32/64 big or little endian.  The field is the CANONICAL gauge field
+ * [t][z][y][x][mu][row][col][re,im] in FP64 (V*72 doubles), e.g. for
+ * cps_hip_gauge_import_canonical.  The reader checks the CHECKSUM (32-bit
+ * sum of the data words, fpconv.C:320-361) and the PLAQUETTE to relative
+ * precision chkprec (the reference's default is 0.01); the link trace is
+ * reported, not enforced (ReadLatticePar.C:278-281).  The writer stores
+ * IEEE32BIG or IEEE64BIG. */
+typedef struct {
+  int dims[4];
+  int recon_row_3;      /* 1: two rows per link stored (4D_SU3_GAUGE) */
+  int bytes_per_real;   /* 4 or 8 */
+  int big_endian;
+  unsigned int checksum_header, checksum_data;
+  double plaquette_header, plaquette_data;
+  double link_trace_header, link_trace_data;
+} cps_hip_nersc_info;
+
+int cps_hip_nersc_read_header(const char *path, cps_hip_nersc_info *info);
+int cps_hip_nersc_read(const char *path, double *u_canon, cps_hip_nersc_info *info, double chkprec);
+int cps_hip_nersc_write(const char *path, const int dims[4], const double *u_canon, int recon_row_3, int bytes_per_real,
+                        const char *ensemble_id, int sequence_number);
 
 /* ---- point-source propagator (QPropW::Run, QPropW.C:566-613) -------------
  * The 12 spin-colour point sources at site x0 (FermionVectorTp::
