@@ -62,6 +62,9 @@ struct HopParams {
   // both parities), perm_S = Vcb / 8 sites per spatial block
   const uint32_t* perm;
   int perm_S;
+  // launch geometry set by the launcher for the chosen block size: blocks
+  // per rhs, blocks per (spatial block, rhs), log2(nb) or -1
+  int bpr, bpr_b, nb_shift;
   int clov;
   int rhs_per_set;
   CloverRef clover;

@@ -9,10 +9,17 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
   const int bs = P.block;
   const int64_t nblk = (nelem + bs - 1) / bs;
   dim3 grid((unsigned)nblk), block(bs);
+  // 32-bit block decomposition (hop_decode): per-rhs counts below 2^31
+  const int64_t per_rhs = (int64_t)P.nsites_out * P.nb;
+  if (per_rhs % bs || per_rhs >= (int64_t(1) << 31) || nblk >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+  HopParams Q = P;
+  Q.bpr = (int)(per_rhs / bs);
+  Q.bpr_b = (int)(((int64_t)P.perm_S * P.nb) / bs);
+  Q.nb_shift = (P.nb & (P.nb - 1)) == 0 ? __builtin_ctz((unsigned)P.nb) : -1;
   const bool nt = (P.flags & HOP_NT_STORE) != 0;
 #define CPSHIP_HOP_LAUNCH(DAG, KIND, SW)                                                                      \
-  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, true, CLOV_NONE, SW>), grid, block, 0, st, P); \
-  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_NONE, SW>), grid, block, 0, st, P);
+  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, true, CLOV_NONE, SW>), grid, block, 0, st, Q); \
+  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_NONE, SW>), grid, block, 0, st, Q);
   if (P.kind == KIND_DWF) {
     if (P.flags & HOP_SITEWAVE) return hipErrorInvalidValue;
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_DWF, 0) } else { CPSHIP_HOP_LAUNCH(0, KIND_DWF, 0) }

@@ -559,30 +559,36 @@ __device__ __forceinline__ HopElem hop_decode_sitewave(const HopParams& P, int64
   return E;
 }
 
+// Block-uniform quantities (rhs, sub-block) are computed with 32-bit
+// arithmetic on the block index (scalar registers); per thread only the
+// element within the sub-block is split into (site, j), with a shift when
+// nb is a power of two.  Counts per rhs stay below 2^31 (checked on the
+// host).
 template <int KIND>
-__device__ __forceinline__ HopElem hop_decode(const HopParams& P, int64_t sb, int64_t nsub) {
+__device__ __forceinline__ HopElem hop_decode(const HopParams& P, int64_t sb64, int64_t nsub64) {
   HopElem E;
-  const int64_t per_rhs = (int64_t)P.nsites_out * P.nb;
+  const int sb = (int)sb64, nsub = (int)nsub64;
+  const int el_base = threadIdx.x;
   if (KIND == KIND_WILSON && (P.flags & HOP_SPATIAL)) {
-    // sb = (spatial block b = XCD, rhs, sub-block within (b, rhs)); all of it
-    // is uniform over the block (scalar arithmetic)
-    const int64_t per_b = nsub >> 3;
-    const int b = (int)(sb / per_b);
-    const int64_t n_in = sb - (int64_t)b * per_b;
-    const int64_t bpr = ((int64_t)P.perm_S * P.nb) / blockDim.x;   // sub-blocks per (b, rhs)
-    E.rhs = (int)(n_in / bpr);
-    const int64_t blk = n_in - (int64_t)E.rhs * bpr;
-    const int64_t el = blk * blockDim.x + threadIdx.x;
-    const int q = (int)(el / P.nb);
-    E.j = (int)(el - (int64_t)q * P.nb);
+    // sb = (spatial block b = XCD, rhs, sub-block within (b, rhs))
+    const int per_b = nsub >> 3;
+    const int b = sb / per_b;
+    const int n_in = sb - b * per_b;
+    const int bpr = P.bpr_b;   // sub-blocks per (b, rhs)
+    E.rhs = n_in / bpr;
+    const int blk = n_in - E.rhs * bpr;
+    const int el = blk * (int)blockDim.x + el_base;
+    const int q = P.nb_shift >= 0 ? (el >> P.nb_shift) : el / P.nb;
+    E.j = el - q * P.nb;
     E.site = (int)P.perm[(int64_t)b * P.perm_S + q];
     E.pslot = (int64_t)E.rhs * (bpr << 3) + (int64_t)b * bpr + blk;
   } else {
-    const int64_t e = sb * blockDim.x + threadIdx.x;   // element within this launch
-    E.rhs = (int)(e / per_rhs);
-    const int64_t rem = e - (int64_t)E.rhs * per_rhs;
-    E.site = (int)(rem / P.nb);
-    E.j = (int)(rem - (int64_t)E.site * P.nb);
+    const int bpr = P.bpr;     // blocks per rhs
+    E.rhs = sb / bpr;
+    const int blk = sb - E.rhs * bpr;
+    const int el = blk * (int)blockDim.x + el_base;
+    E.site = P.nb_shift >= 0 ? (el >> P.nb_shift) : el / P.nb;
+    E.j = el - E.site * P.nb;
     E.pslot = sb;
   }
   return E;
