@@ -26,6 +26,10 @@ enum HopFlags {
                           // kind, nb divides 64, no partials): the site's
                           // neighbour entries and links are wave-uniform
                           // (scalar loads), only the spinors are per lane
+  HOP_LDS_LINKS = 16,     // half-precision spinors, nb >= 8: the block stages
+                          // the 8 links of each of its sites in LDS once (f16
+                          // words, third row rebuilt), and every thread keeps
+                          // all 8 neighbour spinors in flight
 };
 
 struct HopParams {
@@ -65,6 +69,9 @@ struct HopParams {
   // launch geometry set by the launcher for the chosen block size: blocks
   // per rhs, blocks per (spatial block, rhs), log2(nb) or -1
   int bpr, bpr_b, nb_shift;
+  // HOP_LDS_LINKS: sites per block the LDS area is sized for (set by the
+  // launcher)
+  int lds_sites;
   int clov;
   int rhs_per_set;
   CloverRef clover;
@@ -88,6 +95,16 @@ inline int link_storage_of(int hprec) {
     default: return 5;           // LINK_B18
   }
 }
+
+// HOP_LDS_LINKS layout: words 0-7 of the link of (block site sl, direction d)
+// at sl*LDS_SITE_WORDS + 8d (16-byte aligned; consecutive sites start 4
+// banks apart, so the few sites one wavefront reads do not conflict), word 8
+// at lds_sites*LDS_SITE_WORDS + 8 sl + d.
+constexpr int LDS_SITE_WORDS = 68;
+
+// sites of a block of bs consecutive elements, nb elements per site (at most)
+inline int lds_sites_per_block(int bs, int nb) { return bs % nb == 0 ? bs / nb : (bs - 1) / nb + 2; }
+inline size_t lds_link_bytes(int sites) { return (size_t)sites * (LDS_SITE_WORDS + 8) * 4; }
 
 // block size dividing n (a power of two, at most 256)
 inline int pick_block(long long n) {

@@ -17,10 +17,27 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
   Q.bpr_b = (int)(((int64_t)P.perm_S * P.nb) / bs);
   Q.nb_shift = (P.nb & (P.nb - 1)) == 0 ? __builtin_ctz((unsigned)P.nb) : -1;
   const bool nt = (P.flags & HOP_NT_STORE) != 0;
+  // HOP_LDS_LINKS: half-precision spinors, one link per thread to stage
+  const bool lds = (P.flags & HOP_LDS_LINKS) != 0;
+  size_t shm = 0;
+  if (lds) {
+    Q.lds_sites = lds_sites_per_block(bs, P.nb);
+    if (!std::is_same<SIN, SpinH>::value || (P.flags & HOP_SITEWAVE) || 8 * Q.lds_sites > bs)
+      return hipErrorInvalidValue;
+    shm = lds_link_bytes(Q.lds_sites);
+  }
 #define CPSHIP_HOP_LAUNCH(DAG, KIND, SW)                                                                      \
-  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, true, CLOV_NONE, SW>), grid, block, 0, st, Q); \
-  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_NONE, SW>), grid, block, 0, st, Q);
-  if (P.kind == KIND_DWF) {
+  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, true, CLOV_NONE, SW>), grid, block, shm, st, Q); \
+  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_NONE, SW>), grid, block, shm, st, Q);
+  if (lds) {
+    if constexpr (std::is_same<SIN, SpinH>::value) {
+      if (P.kind == KIND_DWF) {
+        if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_DWF, 2) } else { CPSHIP_HOP_LAUNCH(0, KIND_DWF, 2) }
+      } else {
+        if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_WILSON, 2) } else { CPSHIP_HOP_LAUNCH(0, KIND_WILSON, 2) }
+      }
+    }
+  } else if (P.kind == KIND_DWF) {
     if (P.flags & HOP_SITEWAVE) return hipErrorInvalidValue;
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_DWF, 0) } else { CPSHIP_HOP_LAUNCH(0, KIND_DWF, 0) }
   } else if (P.flags & HOP_SITEWAVE) {

@@ -26,12 +26,24 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
   Q.bpr_b = (int)(((int64_t)P.perm_S * P.nb) / bs);
   Q.nb_shift = (P.nb & (P.nb - 1)) == 0 ? __builtin_ctz((unsigned)P.nb) : -1;
   const bool nt = (P.flags & HOP_NT_STORE) != 0;
+  // HOP_LDS_LINKS: half-precision spinors, one link per thread to stage
+  const bool lds = (P.flags & HOP_LDS_LINKS) != 0;
+  size_t shm = 0;
+  if (lds) {
+    Q.lds_sites = lds_sites_per_block(bs, P.nb);
+    if (!std::is_same<SIN, SpinH>::value || (P.flags & HOP_SITEWAVE) || 8 * Q.lds_sites > bs)
+      return hipErrorInvalidValue;
+    shm = lds_link_bytes(Q.lds_sites);
+  }
 #define CPSHIP_HOP_LAUNCH2(DAG, CLOV, SW)                                                                        \
-  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, true, CLOV, SW>), grid, block, 0, st, Q);  \
-  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, false, CLOV, SW>), grid, block, 0, st, Q);
-#define CPSHIP_HOP_LAUNCH(DAG, CLOV)                                   \
-  if (P.flags & HOP_SITEWAVE) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 1) }     \
-  else { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 0) }
+  if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, true, CLOV, SW>), grid, block, shm, st, Q); \
+  else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, false, CLOV, SW>), grid, block, shm, st, Q);
+#define CPSHIP_HOP_LAUNCH(DAG, CLOV)                                                              \
+  if (P.flags & HOP_SITEWAVE) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 1) }                                \
+  else if (lds) {                                                                                 \
+    if constexpr (std::is_same<SIN, SpinH>::value && CLOV == CLOV_S5X) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 2) } \
+    else return hipErrorInvalidValue;                                                             \
+  } else { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 0) }
   if (P.clov == CLOV_POST) {
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, CLOV_POST) } else { CPSHIP_HOP_LAUNCH(0, CLOV_POST) }
   } else if (P.clov == CLOV_X) {

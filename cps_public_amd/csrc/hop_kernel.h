@@ -311,9 +311,10 @@ __device__ __forceinline__ void link_words_f16(const LinkRaw<LK>& r, float sign,
   }
 }
 
-template <int D, typename LK, int DAG>
-__device__ __forceinline__ void dir_compute_h2(const SpinRaw<SpinH>& sr, const LinkRaw<LK>& lr, float sign,
-                                               cplx<float> acc[12]) {
+// one direction from the link as f16 words (U[3 row + col], third row
+// already rebuilt)
+template <int D, int DAG>
+__device__ __forceinline__ void dir_compute_h2w(const SpinRaw<SpinH>& sr, const uint32_t U[9], cplx<float> acc[12]) {
   constexpr int MU = D & 3;
   constexpr int S = (DAG == 0) ? 1 : -1;
   constexpr int A = (D < 4) ? -S : S;
@@ -329,8 +330,6 @@ __device__ __forceinline__ void dir_compute_h2(const SpinRaw<SpinH>& sr, const L
     hb[3 + c] = proj_word<PA::a1, V1>(p[3 + c], p[3 * PA::q1 + c]);
     hx[3 + c] = proj_word<PA::a1, V2>(p[3 + c], p[3 * PA::q1 + c]);
   }
-  uint32_t U[9];
-  link_words_f16<LK>(lr, sign, U);
   cplx<float> w[6];
 #pragma unroll
   for (int s = 0; s < 2; s++)
@@ -348,6 +347,14 @@ __device__ __forceinline__ void dir_compute_h2(const SpinRaw<SpinH>& sr, const L
       w[3 * s + r] = cmk<float>(re, im);
     }
   reconstruct_scaled<MU, A>(acc, w, sr.n);
+}
+
+template <int D, typename LK, int DAG>
+__device__ __forceinline__ void dir_compute_h2(const SpinRaw<SpinH>& sr, const LinkRaw<LK>& lr, float sign,
+                                               cplx<float> acc[12]) {
+  uint32_t U[9];
+  link_words_f16<LK>(lr, sign, U);
+  dir_compute_h2w<D, DAG>(sr, U, acc);
 }
 
 template <int D, typename LK, int DAG>
@@ -466,6 +473,31 @@ __device__ __forceinline__ void hop_batch(const HopParams& P, const Nbr& N, int 
 #undef CPSHIP_DIR
 }
 
+// HOP_LDS_LINKS: the spinor loads of directions D0 .. D0+NB-1 ...
+template <int D0, int NB>
+__device__ __forceinline__ void hop_lds_load(const HopParams& P, const Nbr& N, int64_t in_base, int j,
+                                             SpinRaw<SpinH> sr[NB]) {
+#pragma unroll
+  for (int k = 0; k < NB; k++) raw_load<SpinH>(sr[k], P.in, in_base + (int64_t)N.idx[D0 + k] * P.nb + j);
+}
+
+// ... and their products with the links staged in LDS (block site sl)
+template <int D0, int NB, int DAG>
+__device__ __forceinline__ void hop_lds_compute(const HopParams& P, const uint32_t* lds, int sl,
+                                                const SpinRaw<SpinH> sr[NB], cplx<float> acc[12]) {
+#define CPSHIP_DIR(K)                                                                             \
+  if constexpr ((K) < NB) {                                                                       \
+    constexpr int D = D0 + (K);                                                                   \
+    const uint4* w = reinterpret_cast<const uint4*>(lds + sl * LDS_SITE_WORDS + 8 * D);           \
+    const uint4 a = w[0], b = w[1];                                                               \
+    const uint32_t U[9] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w,                                \
+                           lds[P.lds_sites * LDS_SITE_WORDS + 8 * sl + D]};                       \
+    dir_compute_h2w<D, DAG>(sr[K], U, acc);                                                       \
+  }
+  CPSHIP_DIR(0) CPSHIP_DIR(1) CPSHIP_DIR(2) CPSHIP_DIR(3) CPSHIP_DIR(4) CPSHIP_DIR(5) CPSHIP_DIR(6) CPSHIP_DIR(7)
+#undef CPSHIP_DIR
+}
+
 template <typename R>
 __device__ __forceinline__ R wave_sum(R v) {
 #pragma unroll
@@ -513,6 +545,11 @@ __device__ __forceinline__ int64_t logical_block(int64_t bid, int64_t nblk, int 
 #ifndef CPSHIP_HOP_WAVES_H
 #define CPSHIP_HOP_WAVES_H 3
 #endif
+// spinor load batch of the LDS-link mapping (the links no longer take
+// registers, so all 8 directions can be in flight)
+#ifndef CPSHIP_HOP_BATCH_LDS
+#define CPSHIP_HOP_BATCH_LDS 8
+#endif
 template <typename S> struct HopWaves;
 template <> struct HopWaves<SpinD> { static constexpr int value = CPSHIP_HOP_WAVES_D; };
 template <> struct HopWaves<SpinF> { static constexpr int value = CPSHIP_HOP_WAVES_F; };
@@ -523,6 +560,10 @@ template <> struct HopWaves<SpinH> { static constexpr int value = CPSHIP_HOP_WAV
 struct HopElem {
   int rhs, site, j;
   int64_t pslot;   // rhs * blocks_per_rhs + sub-block within the rhs
+  // site range of the block (element mapping): sites qb .. qb+nsl-1 in site
+  // order (through the spatial block b's permutation for HOP_SPATIAL); the
+  // element's site is qb + sl
+  int qb, nsl, sl, b;
 };
 
 // HOP_SITEWAVE decode (Wilson kind): sub-block sb = (spatial block b | -,
@@ -556,7 +597,21 @@ __device__ __forceinline__ HopElem hop_decode_sitewave(const HopParams& P, int64
   E.site = __builtin_amdgcn_readfirstlane(site);
   E.j = lane - (lane / P.nb) * P.nb;
   E.pslot = 0;   // no partials in this mode
+  E.qb = E.nsl = E.sl = E.b = 0;
   return E;
+}
+
+// the block's site range: first element el0, the element's own site index q
+__device__ __forceinline__ void hop_block_sites(const HopParams& P, int el0, int q, HopElem& E) {
+  const int el1 = el0 + (int)blockDim.x - 1;
+  if (P.nb_shift >= 0) {
+    E.qb = el0 >> P.nb_shift;
+    E.nsl = (el1 >> P.nb_shift) - E.qb + 1;
+  } else {
+    E.qb = el0 / P.nb;
+    E.nsl = el1 / P.nb - E.qb + 1;
+  }
+  E.sl = q - E.qb;
 }
 
 // Block-uniform quantities (rhs, sub-block) are computed with 32-bit
@@ -577,21 +632,67 @@ __device__ __forceinline__ HopElem hop_decode(const HopParams& P, int64_t sb64, 
     const int bpr = P.bpr_b;   // sub-blocks per (b, rhs)
     E.rhs = n_in / bpr;
     const int blk = n_in - E.rhs * bpr;
-    const int el = blk * (int)blockDim.x + el_base;
+    const int el0 = blk * (int)blockDim.x;
+    const int el = el0 + el_base;
     const int q = P.nb_shift >= 0 ? (el >> P.nb_shift) : el / P.nb;
     E.j = el - q * P.nb;
     E.site = (int)P.perm[(int64_t)b * P.perm_S + q];
     E.pslot = (int64_t)E.rhs * (bpr << 3) + (int64_t)b * bpr + blk;
+    E.b = b;
+    hop_block_sites(P, el0, q, E);
   } else {
     const int bpr = P.bpr;     // blocks per rhs
     E.rhs = sb / bpr;
     const int blk = sb - E.rhs * bpr;
-    const int el = blk * (int)blockDim.x + el_base;
+    const int el0 = blk * (int)blockDim.x;
+    const int el = el0 + el_base;
     E.site = P.nb_shift >= 0 ? (el >> P.nb_shift) : el / P.nb;
     E.j = el - E.site * P.nb;
     E.pslot = sb;
+    E.b = 0;
+    hop_block_sites(P, el0, E.site, E);
   }
   return E;
+}
+
+// LDS staging of the links (HOP_LDS_LINKS): thread t < 8 nsl loads the link
+// of direction d = t % 8 of block site sl = t / 8, as hop_batch would, and
+// stores it in the f16 word form of the dot2 product (third row rebuilt,
+// boundary sign applied).  Every thread of the block calls it; the caller
+// synchronises the block before reading.
+template <typename LK, int KIND>
+__device__ __forceinline__ void lds_fill_links(const HopParams& P, const HopElem& E, uint32_t* lds) {
+  const int t = threadIdx.x;
+  if (t >= 8 * E.nsl) return;
+  const int sl = t >> 3, d = t & 7, mu = d & 3;
+  const int q = E.qb + sl;
+  const Geom& g = P.geom;
+  const int site = (KIND == KIND_WILSON && (P.flags & HOP_SPATIAL)) ? (int)P.perm[(int64_t)E.b * P.perm_S + q] : q;
+  int out_par, site_cb;
+  if (KIND == KIND_DWF) {
+    out_par = site >= g.Vcb ? 1 : 0;
+    site_cb = site - out_par * g.Vcb;
+  } else {
+    out_par = P.out_parity;
+    site_cb = site;
+  }
+  const int64_t ns = (int64_t)out_par * g.Vcb + site_cb;
+  int64_t lidx;
+  if (d < 4) {
+    lidx = (int64_t)(out_par * 4 + mu) * g.Vcb + site_cb;
+  } else {
+    const uint32_t nidx = reinterpret_cast<const uint32_t*>(P.nbr)[(2 * (int64_t)g.Vcb + ns) * 4 + mu];
+    lidx = (int64_t)((1 - out_par) * 4 + mu) * g.Vcb + nidx;
+  }
+  const float sign = (P.link_bc_sign && ((P.nbr_neg[ns] >> d) & 1u)) ? -1.f : 1.f;
+  LinkRaw<LK> lr;
+  link_raw_load<LK>(lr, P.gauge, lidx);
+  uint32_t U[9];
+  link_words_f16<LK>(lr, sign, U);
+  uint4* w = reinterpret_cast<uint4*>(lds + sl * LDS_SITE_WORDS + 8 * d);
+  w[0] = make_uint4(U[0], U[1], U[2], U[3]);
+  w[1] = make_uint4(U[4], U[5], U[6], U[7]);
+  lds[P.lds_sites * LDS_SITE_WORDS + t] = U[8];
 }
 
 // neighbour indices and boundary bits of the element's site, precomputed per
@@ -618,33 +719,47 @@ __device__ __forceinline__ Nbr hop_nbr(const HopParams& P, const HopElem& E) {
 
 // One output element: the hop, the fifth-dimension term (DWF), the epilogue
 // and the store; returns the element's |out|^2 when partials are requested.
-template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV>
-__device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, const Nbr& N) {
+template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV, int LDSL>
+__device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, const Nbr& N, uint32_t* lds) {
   const int rhs = E.rhs, site = E.site, j = E.j;
   const int64_t per_rhs = (int64_t)P.nsites_out * P.nb;
   const int64_t rem = (int64_t)site * P.nb + j;
   const bool valid = rhs < P.nrhs;
   R nrm = R(0);
-  if (valid) {
-    const Geom& g = P.geom;
-    int out_par, site_cb;
-    if (KIND == KIND_DWF) {       // fields hold both 4-D parities
-      out_par = site >= g.Vcb ? 1 : 0;
-      site_cb = site - out_par * g.Vcb;
-    } else {
-      out_par = P.out_parity;
-      site_cb = site;
-    }
-
-    int64_t in_base;
-    if (KIND == KIND_DWF) in_base = ((int64_t)rhs * 2 * g.Vcb + (int64_t)(1 - out_par) * g.Vcb) * P.nb;
-    else in_base = (int64_t)rhs * g.Vcb * P.nb;
-
-    cplx<R> acc[12];
+  const Geom& g = P.geom;
+  int out_par, site_cb;
+  if (KIND == KIND_DWF) {       // fields hold both 4-D parities
+    out_par = site >= g.Vcb ? 1 : 0;
+    site_cb = site - out_par * g.Vcb;
+  } else {
+    out_par = P.out_parity;
+    site_cb = site;
+  }
+  int64_t in_base;
+  if (KIND == KIND_DWF) in_base = ((int64_t)rhs * 2 * g.Vcb + (int64_t)(1 - out_par) * g.Vcb) * P.nb;
+  else in_base = (int64_t)rhs * g.Vcb * P.nb;
+  cplx<R> acc[12];
 #pragma unroll
-    for (int k = 0; k < 12; k++) acc[k] = cmk<R>(R(0), R(0));
-    constexpr int NB = HopBatch<SIN>::value;
-    if constexpr (NB == 8) {
+  for (int k = 0; k < 12; k++) acc[k] = cmk<R>(R(0), R(0));
+  if constexpr (LDSL) {
+    // the spinor loads go out before the block waits for its links
+    constexpr int NBL = CPSHIP_HOP_BATCH_LDS;
+    SpinRaw<SpinH> sr[NBL];
+    if (valid) hop_lds_load<0, NBL>(P, N, in_base, j, sr);
+    lds_fill_links<LK, KIND>(P, E, lds);
+    __syncthreads();
+    if (valid) {
+      hop_lds_compute<0, NBL, DAG>(P, lds, E.sl, sr, acc);
+      if constexpr (NBL == 4) {
+        hop_lds_load<4, 4>(P, N, in_base, j, sr);
+        hop_lds_compute<4, 4, DAG>(P, lds, E.sl, sr, acc);
+      }
+    }
+  }
+  if (valid) {
+    constexpr int NB = LDSL ? 0 : HopBatch<SIN>::value;
+    if constexpr (NB == 0) {
+    } else if constexpr (NB == 8) {
       hop_batch<0, 8, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
     } else if constexpr (NB == 4) {
       hop_batch<0, 4, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
@@ -766,14 +881,17 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
 
 // SW = 0: one output element per thread in element order; SW = 1
 // (HOP_SITEWAVE): one site per wavefront, so the site's neighbour entries
-// and links are read with scalar loads.
+// and links are read with scalar loads; SW = 2 (HOP_LDS_LINKS): element
+// order, links staged per block in LDS.
 template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV, int SW>
 __global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) {
+  extern __shared__ uint4 hop_lds_area[];   // HOP_LDS_LINKS (SW = 2): the block's links
   const int64_t nsub = (int64_t)gridDim.x;
   const int64_t hb = logical_block(blockIdx.x, gridDim.x, !(P.flags & HOP_NO_XCD_REMAP));
-  const HopElem E = SW ? hop_decode_sitewave(P, hb, nsub) : hop_decode<KIND>(P, hb, nsub);
+  const HopElem E = SW == 1 ? hop_decode_sitewave(P, hb, nsub) : hop_decode<KIND>(P, hb, nsub);
   const Nbr N = hop_nbr<KIND>(P, E);
-  const R nrm = hop_element<SIN, SOUT, LK, R, DAG, KIND, NT, CLOV>(P, E, N);
+  const R nrm = hop_element<SIN, SOUT, LK, R, DAG, KIND, NT, CLOV, SW == 2>(
+      P, E, N, reinterpret_cast<uint32_t*>(hop_lds_area));
   if (P.partial) block_partial<R>(nrm, P.partial, E.pslot);
 }
 

@@ -318,6 +318,24 @@ static bool sitewave_ok(Lattice& L, const HopParams& P, int b, bool spatial) {
   return true;
 }
 
+// whether a launch may use HOP_LDS_LINKS at block size b (half-precision
+// spinors, at most one link per thread to stage)
+static bool lds_links_ok(const HopParams& P, int b) {
+  const bool half = P.prec == PREC_H_H12 || P.prec == PREC_H_B12 || P.prec == PREC_H_H18 || P.prec == PREC_H_B18;
+  if (!half || (P.clov != CLOV_NONE && P.clov != CLOV_S5X)) return false;
+  return 8 * lds_sites_per_block(b, P.nb) <= b;
+}
+
+// CPS_HIP_TUNE_REPORT=1 prints each tuning decision to stderr, =2 also the
+// time of every candidate
+static int tune_report() {
+  static const int v = [] {
+    const char* e = getenv("CPS_HIP_TUNE_REPORT");
+    return e ? atoi(e) : 0;
+  }();
+  return v;
+}
+
 int tuned_launch(Lattice& L, HopParams& P) {
   P.nbr = (const uint4*)L.nbr.p;
   P.nbr_neg = (const uint32_t*)L.nbr_neg.p;
@@ -328,7 +346,8 @@ int tuned_launch(Lattice& L, HopParams& P) {
   if (P.kind == KIND_WILSON && L.perm_state >= 0 && L.g.Vcb % 8 == 0 && L.ensure_perm() == 0)
     P.perm = (const uint32_t*)L.perm.p;
   int mode = 0;
-  if (const char* m = getenv("CPS_HIP_HOP_MODE")) mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : 0;
+  if (const char* m = getenv("CPS_HIP_HOP_MODE"))
+    mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3 : 0;
   HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr,
                  P.clov, {L.g.L[0], L.g.L[1], L.g.L[2], L.g.L[3]}, mode};
   // the winners are kept per process (every handle, engine and thread of the
@@ -360,14 +379,18 @@ int tuned_launch(Lattice& L, HopParams& P) {
       CPSHIP_CHECK(hipEventCreate(&e1));
       float best_ms = 1e30f;
       for (int b : blocks)
-        for (int f = 0; f < 16; f++) {
+        for (int f = 0; f < 32; f++) {
           if (f & HOP_SPATIAL) {
             if ((f & HOP_NO_XCD_REMAP) || !spatial_ok(L, P, b)) continue;
           }
           if ((f & HOP_SITEWAVE) && !sitewave_ok(L, P, b, (f & HOP_SPATIAL) != 0)) continue;
-          // CPS_HIP_HOP_MODE=sitewave / element restricts the candidates
-          // (where site-per-wave is possible) to that thread mapping
-          if (mode && sitewave_ok(L, P, b, false) && ((f & HOP_SITEWAVE) != 0) != (mode == 2)) continue;
+          if ((f & HOP_LDS_LINKS) && ((f & HOP_SITEWAVE) || !lds_links_ok(P, b))) continue;
+          // CPS_HIP_HOP_MODE=sitewave / lds / element restricts the
+          // candidates (where that mapping is possible) to that thread mapping
+          if ((mode == 1 || mode == 2) && sitewave_ok(L, P, b, false) && ((f & HOP_SITEWAVE) != 0) != (mode == 2))
+            continue;
+          if ((mode == 1 || mode == 3) && lds_links_ok(P, b) && ((f & HOP_LDS_LINKS) != 0) != (mode == 3)) continue;
+          if (mode == 3 && (f & HOP_SITEWAVE)) continue;
           if (P.partial && (f & HOP_SPATIAL) != (best.flags & HOP_SPATIAL)) continue;
           P.block = b;
           P.flags = f;
@@ -378,6 +401,8 @@ int tuned_launch(Lattice& L, HopParams& P) {
           CPSHIP_CHECK(hipEventSynchronize(e1));
           float ms = 0;
           CPSHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          if (tune_report() >= 2)
+            fprintf(stderr, "cps_hip tune:   block %d flags %d: %.3f ms\n", b, f, ms / 3);
           if (ms < best_ms) {
             best_ms = ms;
             best.block = b;
@@ -387,11 +412,7 @@ int tuned_launch(Lattice& L, HopParams& P) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       best.ms = best_ms / 3;
-      static const int report = [] {
-        const char* e = getenv("CPS_HIP_TUNE_REPORT");
-        return e && *e == '1';
-      }();
-      if (report)
+      if (tune_report() >= 1)
         fprintf(stderr, "cps_hip tune: kind %d prec %d dag %d nb %d nsites %d nrhs %d clov %d -> block %d flags %d (%.3f ms)\n",
                 P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.clov, best.block, best.flags, best.ms);
     }

@@ -651,6 +651,32 @@ def test_sitewave_mapping_bit_identical(oracle, monkeypatch, prec):
             assert rel(res["sitewave"][0][k * n:(k + 1) * n], ref) < 1e-13
 
 
+@pytest.mark.parametrize("kind,ls,nrhs", [("mobius", 16, 2), ("mobius", 12, 1), ("mobius", 8, 1), ("dwf", 16, 2)])
+def test_lds_links_mapping_bit_identical(monkeypatch, kind, ls, nrhs):
+    """Half-precision MdagM (both hops, the first with its |M x|^2 partial
+    sums) with the links staged in LDS gives the same bits as the element
+    mapping that loads them per thread; Ls = 12 puts sites across block
+    boundaries (nb = 12 does not divide the block)."""
+    L = [4, 4, 4, 8]
+    u = wilson_gauge(L, 0.2, 230)
+    if kind == "mobius":
+        op = C.make_op(C.OP_MOBIUS, mass=0.05, m5=1.8, b=1.5, c=0.5)
+    else:
+        op = C.make_op(C.OP_DWF, mass=0.05, m5=1.8)
+    vcb = int(np.prod(L)) // 2
+    src = np.concatenate([F.gaussian_fermion(vcb * ls, 231 + k) for k in range(nrhs)])
+    res = {}
+    for mode in ("element", "lds"):
+        monkeypatch.setenv("CPS_HIP_HOP_MODE", mode)
+        lat = C.Lattice(L, ls, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        res[mode] = lat.mdagm_prec(op, src, C.PREC_MIXED_HALF, nrhs=nrhs)
+        lat.close()
+    for a, b_ in zip(res["element"], res["lds"]):
+        assert np.array_equal(a, b_)
+    assert np.abs(res["lds"][0]).max() > 0
+
+
 # ---------------------------------------------------------------------------
 # CANONICAL order on the device: gauge import (Convert + BondCond) and
 # Lattice::FmatInv on CANONICAL vectors
