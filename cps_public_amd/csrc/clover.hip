@@ -283,6 +283,8 @@ int ensure_clover(Lattice& L, int cs) {
 // n kappas, nrhs = n * k and rhs r uses kappa index r / k.
 int clover_sets(Lattice& L, const cps_hip_op* op, int nrhs, int* rhs_per_set) {
   if (L.n_clover == 0) return CPS_HIP_ERR_ARG;
+  // inside a CG with an active-rhs window the set follows the global index
+  if (L.win_total > 0) nrhs = L.win_total;
   if (op->csw != L.clover_csw) return CPS_HIP_ERR_ARG;
   if (L.n_clover == 1) {
     const double k = op_kappa(op);
@@ -359,6 +361,7 @@ int clover_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& ou
   hc.ref.data = L.clover[cs].p;
   hc.ref.plane = L.g.Vcb;
   hc.rhs_per_set = per;
+  hc.rhs0 = L.win_total > 0 ? L.win_lo : 0;
   hc.mode = CLOV_POST;
   hc.kscale = nullptr;
   rc = hop_ex(L, op, hprec, t1, in, 0, dag, 1.0, null_field(), 0.0, nullptr, nrhs, &hc);

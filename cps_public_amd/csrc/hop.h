@@ -74,6 +74,7 @@ struct HopParams {
   int lds_sites;
   int clov;
   int rhs_per_set;
+  int rhs0;          // clover set of local rhs r: (rhs0 + r) / rhs_per_set
   CloverRef clover;
   const double* kscale;
 };

@@ -836,7 +836,7 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
         res[c_from_p + k].im += x5 * (cp * hp[k].im);
       }
     } else if constexpr (CLOV != CLOV_NONE) {
-      const int set = rhs / P.rhs_per_set;
+      const int set = (P.rhs0 + rhs) / P.rhs_per_set;
       // (no branch here: a conditional load splits the block and the stencil
       // above it then spills)
       R sc;

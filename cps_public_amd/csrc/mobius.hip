@@ -596,7 +596,7 @@ int mobius_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, c
 int mobius_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
                  const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs) {
   const int st = store_of(hprec);
-  Field& t3 = L.field(st == STORE_D ? S_T3 : S_T3S, st, (int64_t)nrhs * L.g.Vcb * L.ls);
+  Field& t3 = L.field_min(st == STORE_D ? S_T3 : S_T3S, st, (int64_t)nrhs * L.g.Vcb * L.ls);
   if (L.last_rc) return L.last_rc;
   int rc = mobius_m(L, op, hprec, t3.ref(), in, t1, t2, 0, partial, nrhs);
   if (rc) return rc;

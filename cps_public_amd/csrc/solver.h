@@ -84,6 +84,9 @@ struct Lattice {
   bool clover_valid[NCLOVER];
   DevBuf clover_ksq;   // kappa^2 per set (device)
   int n_clover = 0;
+  // active-rhs window of a running CG: its operator calls see local rhs
+  // 0 .. n-1 = global rhs win_lo .. win_lo+n-1 of win_total (0: no window)
+  int win_lo = 0, win_total = 0;
   std::vector<double> clover_kappas;
   double clover_csw = 0;
   int last_rc = 0;
@@ -98,6 +101,9 @@ struct Lattice {
   int ensure_gauge(int link_storage);
   GaugeRef gauge_ref(int link_storage) const;
   Field& field(int slot, int storage, int64_t nelem);
+  // as field(), but an existing buffer of the same storage holding at least
+  // nelem elements is kept (its plane stride is then its own size)
+  Field& field_min(int slot, int storage, int64_t nelem);
   int ensure_partial(int64_t slots);
   long long device_bytes() const;
 };
@@ -146,6 +152,13 @@ struct IterGraph {
     for (int i = 0; i < n; i++) CPSHIP_CHECK(hipGraphLaunch(exec, L.stream));
     return 0;
   }
+  // the launch sequence changed (new operands): capture again on the next run
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
 };
 
 struct CGState {
@@ -167,6 +180,7 @@ struct HopClover {
   CloverRef ref;
   const double* kscale;
   int rhs_per_set;
+  int rhs0;   // global index of local rhs 0 (the CG's active window)
 };
 int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
            double scale, const FieldRef& X, double xcoef, double* partial, int nrhs, const HopClover* hc);

@@ -192,6 +192,12 @@ Field& Lattice::field(int slot, int storage, int64_t nelem) {
   return f;
 }
 
+Field& Lattice::field_min(int slot, int storage, int64_t nelem) {
+  Field& f = scratch[slot];
+  if (f.buf.p && f.storage == storage && f.nelem >= nelem) return f;
+  return field(slot, storage, nelem);
+}
+
 int Lattice::ensure_partial(int64_t slots) {
   return partial.ensure((size_t)std::max<int64_t>(slots, 1) * sizeof(double));
 }
@@ -278,6 +284,7 @@ int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, con
     P.clover = hc->ref;
     P.kscale = hc->kscale;
     P.rhs_per_set = hc->rhs_per_set;
+    P.rhs0 = hc->rhs0;
   }
   rc = tuned_launch(L, P);
   return rc;
@@ -362,6 +369,22 @@ int tuned_launch(Lattice& L, HopParams& P) {
     if (f != tune_cache.end()) {
       chosen = f->second;
       have = true;
+    } else {
+      // the same shape tuned for another rhs count (a CG's active-rhs window
+      // shrinking): reuse that choice where it is valid for this count (for
+      // launches with partials it is the fixed-order choice, which does not
+      // depend on the count)
+      for (const auto& kv : tune_cache) {
+        HopTuneKey k2 = kv.first;
+        k2.nrhs = key.nrhs;
+        if (k2 < key || key < k2) continue;
+        chosen = kv.second;
+        if ((chosen.flags & HOP_SITEWAVE) && !sitewave_ok(L, P, chosen.block, (chosen.flags & HOP_SPATIAL) != 0))
+          chosen.flags &= ~HOP_SITEWAVE;
+        have = true;
+        tune_cache.emplace(key, chosen);
+        break;
+      }
     }
   }
   if (!have) {
@@ -583,6 +606,72 @@ static int auto_check_every(const OpLayout& o, int nrhs) {
   return 8;
 }
 
+// ---------------------------------------------------------------------------
+// Active-rhs window.  Once the rhs at either end of a batch have converged,
+// every kernel of an iteration runs on the contiguous range [lo, hi) of rhs
+// still iterating: the same planes, the data and norm pointers advanced by
+// lo rhs, the per-rhs scalars by lo entries.  Each rhs's arithmetic (and its
+// reduction order) is the same inside or outside the window, so the results
+// do not change; converged rhs in the middle of the range keep their masked
+// updates.  CPS_HIP_RHS_WINDOW=0 keeps the whole batch.
+
+static bool rhs_window_enabled() {
+  const char* e = getenv("CPS_HIP_RHS_WINDOW");   // read per solve
+  return !(e && *e == '0');
+}
+
+static FieldRef rhs_view(const FieldRef& f, int64_t per_rhs, int lo) {
+  FieldRef v = f;
+  if (v.data) v.data = static_cast<char*>(v.data) + (int64_t)lo * per_rhs * 16;   // 16-byte chunks
+  if (v.norm) v.norm += (int64_t)lo * per_rhs;
+  return v;
+}
+
+static CGScalars scalars_view(const CGScalars& s, int lo) {
+  CGScalars v = s;
+  v.rr += lo;
+  v.rr_prev += lo;
+  v.d += lo;
+  v.alpha += lo;
+  v.beta += lo;
+  v.stp += lo;
+  v.src_norm += lo;
+  v.rsd += lo;
+  v.active += lo;
+  v.iters += lo;
+  v.status += lo;
+  if (v.rr_upd) v.rr_upd += lo;
+  if (v.hist) v.hist += (int64_t)lo * v.hist_len;
+  return v;
+}
+
+struct RhsWindow {
+  Lattice& L;
+  bool on;
+  int lo, hi;
+  RhsWindow(Lattice& l, int nrhs) : L(l), on(rhs_window_enabled()), lo(0), hi(nrhs) {
+    L.win_lo = 0;
+    L.win_total = nrhs;
+  }
+  ~RhsWindow() {
+    L.win_lo = 0;
+    L.win_total = 0;
+  }
+  // shrink to the range of flags still set; true if the range changed
+  bool update(const std::vector<int>& live) {
+    if (!on) return false;
+    int a = 0, b = (int)live.size();
+    while (a < b && !live[a]) a++;
+    while (b > a && !live[b - 1]) b--;
+    if (a >= b || (a == lo && b == hi)) return false;
+    lo = a;
+    hi = b;
+    L.win_lo = lo;
+    return true;
+  }
+  int n() const { return hi - lo; }
+};
+
 int cg_double_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int nrhs, const cps_hip_cg_arg* arg,
                      std::vector<int>& iters, std::vector<int>& status, std::vector<double>& true_res,
                      std::vector<double>& src_norm) {
@@ -633,17 +722,23 @@ int cg_double_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int
   };
   Progress prog("fp64");
   IterGraph iter_graph(arg->use_graph != 0);
+  RhsWindow win(L, nrhs);
   while (any_active() && done_iters < max_itr) {
     prog.tick(L, done_iters, act, cg.s.rr);
+    if (win.update(act)) iter_graph.reset();
+    const int nw = win.n();
+    const FieldRef xw = rhs_view(x.ref(), o.per_rhs, win.lo), pw = rhs_view(p.ref(), o.per_rhs, win.lo);
+    const FieldRef rw = rhs_view(r.ref(), o.per_rhs, win.lo), apw = rhs_view(ap.ref(), o.per_rhs, win.lo);
+    const CGScalars sw = scalars_view(cg.s, win.lo);
     const int todo = std::min(chunk, max_itr - done_iters);
     rc = iter_graph.run(L, todo, [&]() -> int {
       // mmp = MdagM p, d = |M p|^2 (inv_cg.C:302)
-      int e = apply_mdagm(L, op, PREC_D_D18, ap.ref(), p.ref(), t1.ref(), t2.ref(), part, nrhs);
+      int e = apply_mdagm(L, op, PREC_D_D18, apw, pw, t1.ref(), t2.ref(), part, nw);
       if (e) return e;
-      CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nrhs, nullptr, RED_CG_ALPHA, cg.s, L.stream));
-      CPSHIP_CHECK(blas_cg_xr_d(x.ref(), p.ref(), r.ref(), ap.ref(), cg.s, o.per_rhs, nrhs, bsb, part, L.stream));
-      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_CG_BETA, cg.s, L.stream));
-      CPSHIP_CHECK(blas_xpay(STORE_D, p.ref(), r.ref(), cg.s.beta, cg.s.active, o.per_rhs, nrhs, bsb, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
+      CPSHIP_CHECK(blas_cg_xr_d(xw, pw, rw, apw, sw, o.per_rhs, nw, bsb, part, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, nullptr, RED_CG_BETA, sw, L.stream));
+      CPSHIP_CHECK(blas_xpay(STORE_D, pw, rw, sw.beta, sw.active, o.per_rhs, nw, bsb, L.stream));
       return 0;
     });
     if (rc) return rc;
@@ -651,6 +746,9 @@ int cg_double_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int
     rc = sync_read(L, act.data(), cg.s.active, nrhs * sizeof(int));
     if (rc) return rc;
   }
+  win.lo = 0;
+  win.hi = nrhs;
+  L.win_lo = 0;
   // true residual (inv_cg.C:403-417)
   rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), x.ref(), t1.ref(), t2.ref(), nullptr, nrhs);
   if (rc) return rc;
@@ -813,7 +911,6 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     std::vector<double> zeros(nrhs, 0.0);
     CPSHIP_CHECK(hipMemcpyAsync(cg.s.beta, zeros.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, L.stream));
   }
-  double* rp = cg.s.alpha;       // <r, p> at a reliable update (alpha is rewritten by the next step)
   DevBuf coef_buf;
   if ((rc = coef_buf.ensure(nrhs * sizeof(double)))) return rc;
   double* coef = (double*)coef_buf.p;
@@ -833,21 +930,28 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   };
   Progress prog("mixed");
   IterGraph iter_graph(arg->use_graph != 0);
+  RhsWindow win(L, nrhs);   // over the rhs not yet converged on the FP64 residual
   while (any(live) && done < max_itr) {
     prog.tick(L, done, live, cg.s.rr);
+    if (win.update(live)) iter_graph.reset();
+    const int lo = win.lo, nw = win.n();
+    const FieldRef xw = rhs_view(x.ref(), o.per_rhs, lo), srcw = rhs_view(src.ref(), o.per_rhs, lo);
+    const FieldRef rw = rhs_view(r.ref(), o.per_rhs, lo), apw = rhs_view(ap.ref(), o.per_rhs, lo);
+    const FieldRef xsw = rhs_view(xs.ref(), o.per_rhs, lo), rsw = rhs_view(rs.ref(), o.per_rhs, lo);
+    const FieldRef psw = rhs_view(ps.ref(), o.per_rhs, lo), apsw = rhs_view(aps.ref(), o.per_rhs, lo);
+    const CGScalars sw = scalars_view(cg.s, lo);
     const int todo = std::min(std::min(chunk, max_itr - done), every - since);
     // one iteration: the previous step's direction update p = r + beta p
     // first, then Ap, alpha, x and r, |r|^2 and beta -- so that a reliable
     // update at a chunk boundary falls between r's update and p's
     rc = iter_graph.run(L, todo, [&]() -> int {
-      CPSHIP_CHECK(blas_xpay_mixed(sps, ps.ref(), svec, rs.ref(), cg.s.beta, cg.s.active, o.per_rhs, nrhs, bsb,
-                                   L.stream));
-      int e = apply_mdagm(L, op, hprec, aps.ref(), ps.ref(), t1s.ref(), t2s.ref(), part, nrhs);
+      CPSHIP_CHECK(blas_xpay_mixed(sps, psw, svec, rsw, sw.beta, sw.active, o.per_rhs, nw, bsb, L.stream));
+      int e = apply_mdagm(L, op, hprec, apsw, psw, t1s.ref(), t2s.ref(), part, nw);
       if (e) return e;
-      CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nrhs, nullptr, RED_CG_ALPHA, cg.s, L.stream));
-      CPSHIP_CHECK(blas_cg_xr_mixed(svec, xs.ref(), sps, ps.ref(), svec, rs.ref(), sps, aps.ref(), cg.s,
-                                    o.per_rhs, nrhs, bsb, part, L.stream));
-      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, nullptr, RED_CG_BETA, cg.s, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
+      CPSHIP_CHECK(blas_cg_xr_mixed(svec, xsw, sps, psw, svec, rsw, sps, apsw, sw, o.per_rhs, nw, bsb, part,
+                                    L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, nullptr, RED_CG_BETA, sw, L.stream));
       return 0;
     });
     if (rc) return rc;
@@ -871,20 +975,22 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     }
     if (since >= every || paused || done >= max_itr) {
       // ---- FP64-critical steps: accumulate the solution, recompute r ----
-      CPSHIP_CHECK(blas_axpy(svec, xs.ref(), STORE_D, x.ref(), nullptr, 1.0, o.per_rhs, nrhs, bsb, L.stream));
-      CPSHIP_CHECK(blas_zero(svec, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
-      rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), x.ref(), t1.ref(), t2.ref(), nullptr, nrhs);
+      // (on the window: the rhs outside it have converged and keep x)
+      CPSHIP_CHECK(blas_axpy(svec, xsw, STORE_D, xw, nullptr, 1.0, o.per_rhs, nw, bsb, L.stream));
+      CPSHIP_CHECK(blas_zero(svec, xsw, o.per_rhs, nw, bsb, L.stream));
+      rc = apply_mdagm(L, op, PREC_D_D18, apw, xw, t1.ref(), t2.ref(), nullptr, nw);
       if (rc) return rc;
-      CPSHIP_CHECK(blas_sub_norm_d(src.ref(), ap.ref(), r.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
-      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, rr_true, RED_PLAIN, cg.s, L.stream));
-      CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), svec, rs.ref(), n, L.stream));
-      CPSHIP_CHECK(blas_redot_mixed(svec, rs.ref(), sps, ps.ref(), o.per_rhs, nrhs, bsb, part, L.stream));
-      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, rp, RED_PLAIN, cg.s, L.stream));
-      hipLaunchKernelGGL(k_mixed_after_update, dim3((nrhs + 63) / 64), dim3(64), 0, L.stream, cg.s, rr_true, rp, coef,
-                         nrhs);
+      CPSHIP_CHECK(blas_sub_norm_d(srcw, apw, rw, o.per_rhs, nw, bsb, part, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, rr_true + lo, RED_PLAIN, sw, L.stream));
+      CPSHIP_CHECK(convert_spinor(STORE_D, rw, svec, rsw, o.per_rhs * nw, L.stream));
+      CPSHIP_CHECK(blas_redot_mixed(svec, rsw, sps, psw, o.per_rhs, nw, bsb, part, L.stream));
+      // <r, p> into alpha (rewritten by the next step)
+      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, sw.alpha, RED_PLAIN, sw, L.stream));
+      hipLaunchKernelGGL(k_mixed_after_update, dim3((nw + 63) / 64), dim3(64), 0, L.stream, sw, rr_true + lo,
+                         (const double*)sw.alpha, coef + lo, nw);
       CPSHIP_CHECK(hipGetLastError());
       // p -= (<r,p>/|r|^2) r for the continuing rhs (coef 0 for the others)
-      CPSHIP_CHECK(blas_axpy(svec, rs.ref(), sps, ps.ref(), coef, 0.0, o.per_rhs, nrhs, bsb, L.stream));
+      CPSHIP_CHECK(blas_axpy(svec, rsw, sps, psw, coef + lo, 0.0, o.per_rhs, nw, bsb, L.stream));
       for (int k = 0; k < nrhs; k++)
         if (live[k]) n_updates[k]++;
       since = 0;
@@ -901,6 +1007,9 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
       }
     }
   }
+  win.lo = 0;
+  win.hi = nrhs;
+  L.win_lo = 0;
   // final FP64 true residual of MdagM (inv_cg.C:403-417)
   rc = apply_mdagm(L, op, PREC_D_D18, ap.ref(), x.ref(), t1.ref(), t2.ref(), nullptr, nrhs);
   if (rc) return rc;

@@ -149,6 +149,55 @@ def test_dwf_multi_rhs_matches_single(dwf4):
         assert rel(batched[k * n:(k + 1) * n], single) < 1e-13
 
 
+@pytest.mark.parametrize("prec", [C.PREC_DOUBLE, C.PREC_MIXED_HALF])
+def test_active_rhs_window_bit_identical(monkeypatch, dwf4, prec):
+    """rhs 0 and 3 start from solutions good to 1e-5 and converge first, so
+    the batch shrinks to rhs 1..2 for the rest of the solve; the solutions and
+    iteration counts are bit-identical to the solve that keeps all 4 rhs."""
+    lat, u, L, ls = dwf4
+    op = C.make_op(C.OP_DWF, mass=0.01, m5=1.8)
+    n = 24 * lat.vcb * ls
+    src = np.concatenate([F.gaussian_fermion(lat.vcb * ls, 70 + k) for k in range(4)])
+    rough, _ = lat.invcg(op, src, C.make_cg_arg(1e-5, 5000), nrhs=4)
+    guess = np.zeros_like(src)
+    for k in (0, 3):
+        guess[k * n:(k + 1) * n] = rough[k * n:(k + 1) * n]
+    arg = C.make_cg_arg(1e-10, 5000, precision=prec)
+    out = {}
+    for w in ("1", "0"):
+        monkeypatch.setenv("CPS_HIP_RHS_WINDOW", w)
+        out[w] = lat.invcg(op, src, arg, guess=guess, nrhs=4)
+    it = [r.iterations for r in out["1"][1]]
+    print("rhs window", prec, it, flush=True)
+    assert it == [r.iterations for r in out["0"][1]]
+    assert max(it[0], it[3]) < min(it[1], it[2])
+    assert np.array_equal(out["1"][0], out["0"][0])
+    assert max(r.true_res for r in out["1"][1]) <= 1e-10
+
+
+def test_active_rhs_window_clover_sets(monkeypatch, clov4):
+    """Two kappas, rhs r on kappa r // 2: with rhs 0 and 3 started near their
+    solutions the window holds rhs 1 (kappa 0) and rhs 2 (kappa 1), so each
+    rhs must keep its own clover set inside the window."""
+    lat, u, L = clov4
+    lat.clover_build(CLOVER_KAPPAS, CLOVER_CSW)
+    op = clover_op(CLOVER_KAPPAS[0])
+    n = 24 * lat.vcb
+    src = np.concatenate([F.gaussian_fermion(lat.vcb, 80 + k) for k in range(4)])
+    rough, _ = lat.invcg(op, src, C.make_cg_arg(1e-5, 5000), nrhs=4)
+    guess = np.zeros_like(src)
+    for k in (0, 3):
+        guess[k * n:(k + 1) * n] = rough[k * n:(k + 1) * n]
+    out = {}
+    for w in ("1", "0"):
+        monkeypatch.setenv("CPS_HIP_RHS_WINDOW", w)
+        out[w] = lat.invcg(op, src, C.make_cg_arg(1e-10, 5000), guess=guess, nrhs=4)
+    it = [r.iterations for r in out["1"][1]]
+    assert it == [r.iterations for r in out["0"][1]]
+    assert max(it[0], it[3]) < min(it[1], it[2])
+    assert np.array_equal(out["1"][0], out["0"][0])
+
+
 def test_invcg_device_matches_host(dwf4):
     """cps_hip_invcg_device (vectors resident in HBM) gives the host entry
     point's result bit for bit, for FP64 and mixed precision, two rhs."""
