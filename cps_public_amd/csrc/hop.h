@@ -57,9 +57,11 @@ struct HopParams {
   // kappa index set = rhs / rhs_per_set at the output checkerboard and kscale
   // a device array (required for CLOV_X)
   // CLOV_S5X epilogue (Wilson kind with nb = Ls, the Moebius layout):
-  // out = xcoef*X + x5coef*(P X) + scale*hop, P the fifth-dimension shift of
-  // the hop's dag with -x5mass at the walls
-  double x5coef, x5mass;
+  // out = xcoef*X + x5coef*(P X) + scale*(hb5*hop + hc5*(P hop)), P the
+  // fifth-dimension shift of the hop's dag with -x5mass at the walls; the
+  // P hop term (hc5 != 0) takes the hop of slots s-1, s+1 from the
+  // neighbouring lanes, so it needs nb a power of two dividing 64
+  double x5coef, x5mass, hb5, hc5;
   const uint4* nbr;         // neighbour table (k_build_nbr)
   const uint32_t* nbr_neg;
   // HOP_SPATIAL: site_cb of logical site b*perm_S + q (the same table for

@@ -14,6 +14,9 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
   if ((P.clov == CLOV_POST || P.clov == CLOV_X) && (P.rhs_per_set < 1 || !P.clover.data)) return hipErrorInvalidValue;
   if (P.clov == CLOV_X && !P.kscale) return hipErrorInvalidValue;
   if (P.clov == CLOV_S5X && (!P.X.data || P.nb < 2)) return hipErrorInvalidValue;
+  // the fused P hop term reads slots s-1, s+1 from lanes of one wavefront
+  if (P.clov == CLOV_S5X && P.hc5 != 0.0 && ((P.nb & (P.nb - 1)) || 64 % P.nb || P.block % P.nb))
+    return hipErrorInvalidValue;
   const int64_t nelem = (int64_t)P.nrhs * P.nsites_out * P.nb;
   const int bs = P.block;
   const int64_t nblk = (nelem + bs - 1) / bs;

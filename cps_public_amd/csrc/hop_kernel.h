@@ -826,14 +826,32 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
       spin_load_half<SOUT, R>(P.X, sbase + sm, c_from_m, hm);
       spin_load_half<SOUT, R>(P.X, sbase + sp, c_from_p, hp);
       const R xc = R(P.xcoef), x5 = R(P.x5coef), sc = R(P.scale);
+      const R sb = R(P.scale * P.hb5);
 #pragma unroll
-      for (int k = 0; k < 12; k++) res[k] = cmk<R>(xc * xv[k].re + sc * acc[k].re, xc * xv[k].im + sc * acc[k].im);
+      for (int k = 0; k < 12; k++) res[k] = cmk<R>(xc * xv[k].re + sb * acc[k].re, xc * xv[k].im + sb * acc[k].im);
 #pragma unroll
       for (int k = 0; k < 6; k++) {
         res[c_from_m + k].re += x5 * (cm * hm[k].re);
         res[c_from_m + k].im += x5 * (cm * hm[k].im);
         res[c_from_p + k].re += x5 * (cp * hp[k].re);
         res[c_from_p + k].im += x5 * (cp * hp[k].im);
+      }
+      if (P.hc5 != 0.0) {
+        // P applied to the hop itself: the hop of slot s-1 (s+1) is the one
+        // of the lane below (above) in the site's aligned group of nb lanes
+        const int lane = (int)(threadIdx.x & 63);
+        const int lm = (j == 0) ? lane + ls - 1 : lane - 1;
+        const int lp = (j == ls - 1) ? lane - (ls - 1) : lane + 1;
+        const R scm = R(P.scale * P.hc5) * cm, scp = R(P.scale * P.hc5) * cp;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          const R mre = __shfl(acc[c_from_m + k].re, lm, 64), mim = __shfl(acc[c_from_m + k].im, lm, 64);
+          const R pre = __shfl(acc[c_from_p + k].re, lp, 64), pim = __shfl(acc[c_from_p + k].im, lp, 64);
+          res[c_from_m + k].re += scm * mre;
+          res[c_from_m + k].im += scm * mim;
+          res[c_from_p + k].re += scp * pre;
+          res[c_from_p + k].im += scp * pim;
+        }
       }
     } else if constexpr (CLOV != CLOV_NONE) {
       const int set = (P.rhs0 + rhs) / P.rhs_per_set;

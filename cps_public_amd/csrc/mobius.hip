@@ -511,9 +511,16 @@ static void mobius_kappas(const cps_hip_op* op, double* kb, double* kc) {
 // 4-D hop of every slice (Wilson kernel, nb = Ls), output parity `target`;
 // with s5x the epilogue adds X + x5coef P X (P of direction dag, -mass walls)
 // and writes the per-block |out|^2 partials when partial is given
+// CPS_HIP_S5_FUSED=0 keeps the dag-1 s-term as its own pass
+static bool s5_fused_enabled() {
+  const char* e = getenv("CPS_HIP_S5_FUSED");
+  return !(e && *e == '0');
+}
+
+// s5x: out = X + x5coef (P X) + scale (hb5 hop + hc5 P hop) (CLOV_S5X)
 static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag, double scale,
                 int nrhs, bool s5x = false, const FieldRef* X = nullptr, double x5coef = 0.0, double mass = 0.0,
-                double* partial = nullptr) {
+                double* partial = nullptr, double hb5 = 1.0, double hc5 = 0.0) {
   cps_hip_op w;
   std::memset(&w, 0, sizeof(w));
   w.kind = CPS_HIP_OP_MOBIUS;   // single-parity layout with nb = Ls
@@ -540,6 +547,8 @@ static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, 
     P.xcoef = 1.0;
     P.x5coef = x5coef;
     P.x5mass = mass;
+    P.hb5 = hb5;
+    P.hc5 = hc5;
   }
   const int lstore = link_storage_of(hprec);
   int rc = L.ensure_gauge(lstore);
@@ -582,7 +591,11 @@ int mobius_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, c
   // dag 1: t1 = -kb * D4dag(cb=1) in ; t2 = M5^-dag (b t1 + c Pdag t1)
   if ((rc = hop4(L, hprec, t1, in, 0, 1, -kb, nrhs))) return rc;
   if ((rc = launch_m5inv(L, hprec, t1, 1, ratio, op->mass, nrhs, &t2, S5_BEFORE, b, c))) return rc;
-  // t1 = kb * D4dag(cb=0) t2 ; out = (b t1 + c Pdag t1) + (in + ratio Pdag in)
+  // t1 = kb * D4dag(cb=0) t2 ; out = (b t1 + c Pdag t1) + (in + ratio Pdag in):
+  // one hop with the s-term fused into its epilogue when the Ls slots of a
+  // site fit a wavefront's lane groups, else a hop and a k_s5 pass
+  if (L.ls <= 64 && 64 % L.ls == 0 && s5_fused_enabled())
+    return hop4(L, hprec, out, t2, 1, 1, kb, nrhs, true, &in, ratio, op->mass, partial, b, c);
   if ((rc = hop4(L, hprec, t1, t2, 1, 1, kb, nrhs))) return rc;
   S5Params P = s5_base(L, op, 1, nrhs);
   P.out = out; P.A = t1; P.a1 = 1; P.b1 = b; P.c1 = c;

@@ -113,12 +113,18 @@ struct Lattice {
 // which cannot happen inside a capture); the same sequence is then captured
 // once and every later iteration is one graph launch.  Only kernel launches
 // are captured (no memset nodes).  All kernel arguments are device pointers
-// or values fixed for the solve, so one capture serves every iteration.
+// or values fixed while the operands stay the same, so one capture serves
+// every iteration until reset() (the CG's active-rhs window shrinking).
 struct IterGraph {
   bool enabled;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  explicit IterGraph(bool on) : enabled(on) {}
+  // CPS_HIP_GRAPH=0 launches every iteration eagerly (e.g. under a
+  // profiler that traces kernels but not graph replays)
+  explicit IterGraph(bool on) : enabled(on) {
+    const char* e = getenv("CPS_HIP_GRAPH");
+    if (e && *e == '0') enabled = false;
+  }
   IterGraph(const IterGraph&) = delete;
   IterGraph& operator=(const IterGraph&) = delete;
   ~IterGraph() {

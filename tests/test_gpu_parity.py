@@ -241,6 +241,31 @@ def test_mobius_m_and_mdagm(oracle, mob4):
     assert abs(dp[0] - dp_ref) / dp_ref < 1e-13
 
 
+@pytest.mark.parametrize("ls,fused", [(4, "1"), (16, "1"), (16, "0"), (12, "1")])
+def test_mobius_mdag_s5_fusion(oracle, monkeypatch, ls, fused):
+    """M^dag with its last fifth-dimension step fused into the hop epilogue
+    (slots s-1, s+1 from neighbouring lanes; Ls 4 and 16), kept as its own
+    pass (CPS_HIP_S5_FUSED=0, and Ls 12 which does not divide a wavefront),
+    against the oracle; the mixed-precision M^dag M agrees with FP64 to the
+    FP16 storage precision either way."""
+    monkeypatch.setenv("CPS_HIP_S5_FUSED", fused)
+    L = [4, 4, 4, 4]
+    u = wilson_gauge(L, 0.2, 240)
+    lat = C.Lattice(L, ls, (0, 0, 0, 1))
+    lat.import_gauge(u)
+    m = oracle.mobius_struct(L, ls, 1.8, 1.5, 0.5)
+    op = C.make_op(C.OP_MOBIUS, mass=0.02, m5=1.8, b=1.5, c=0.5)
+    inp = np.concatenate([F.gaussian_fermion(lat.vcb * ls, 241 + k) for k in range(2)])
+    n = 24 * lat.vcb * ls
+    got = lat.matpc(op, inp, 1, nrhs=2)
+    for k in range(2):
+        assert rel(got[k * n:(k + 1) * n], oracle.mobius_mdag(u, inp[k * n:(k + 1) * n], 0.02, m)) < 1e-13
+    d64, _ = lat.mdagm(op, inp, nrhs=2)
+    dh, _ = lat.mdagm_prec(op, inp, C.PREC_MIXED_HALF, nrhs=2)
+    assert rel(dh, d64) < 5e-3
+    lat.close()
+
+
 def test_mobius_cg(oracle, mob4):
     lat, u, L, ls = mob4
     m = oracle.mobius_struct(L, ls, 1.8, 1.5, 0.5)
