@@ -373,6 +373,7 @@ hipError_t blas_cg_xr_mixed(int sx, const FieldRef& x, int sp, const FieldRef& p
     return hipGetLastError();                                                                           \
   }
   CPSHIP_XR(STORE_F, SpinF, STORE_H, SpinH, STORE_F, SpinF, STORE_H, SpinH)
+  CPSHIP_XR(STORE_H, SpinH, STORE_H, SpinH, STORE_F, SpinF, STORE_H, SpinH)
   CPSHIP_XR(STORE_F, SpinF, STORE_H, SpinH, STORE_H, SpinH, STORE_H, SpinH)
   CPSHIP_XR(STORE_F, SpinF, STORE_F, SpinF, STORE_F, SpinF, STORE_F, SpinF)
   CPSHIP_XR(STORE_H, SpinH, STORE_H, SpinH, STORE_H, SpinH, STORE_H, SpinH)

@@ -804,13 +804,16 @@ void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp
   }
 }
 
-static bool sloppy_vectors_half() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("CPS_HIP_SLOPPY_VECTORS");
-    v = (e && std::strcmp(e, "half") == 0) ? 1 : 0;
-  }
-  return v == 1;
+// Storage of the sloppy solution xs and residual rs with half-precision
+// sloppy spinors (CPS_HIP_SLOPPY_VECTORS, read per solve): "single" both FP32,
+// "xhalf" xs FP16 (per-site norms) and rs FP32, "half" both FP16.
+static void sloppy_vector_storage(int sps, int* xst, int* rst) {
+  *xst = *rst = STORE_F;
+  if (sps != STORE_H) return;
+  const char* e = getenv("CPS_HIP_SLOPPY_VECTORS");
+  if (!e) return;
+  if (std::strcmp(e, "half") == 0) *xst = *rst = STORE_H;
+  else if (std::strcmp(e, "xhalf") == 0) *xst = STORE_H;
 }
 
 static void mixed_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp_store, int* hprec) {
@@ -866,8 +869,9 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   // sloppy solution and residual: FP32 by default; with half-precision
   // sloppy storage and CPS_HIP_SLOPPY_VECTORS=half they are FP16 too (less
   // BLAS traffic, more reliance on the reliable updates)
-  const int svec = (sps == STORE_H && sloppy_vectors_half()) ? STORE_H : STORE_F;
-  Field& xs = L.field(S_XS, svec, n);
+  int xvec, svec;
+  sloppy_vector_storage(sps, &xvec, &svec);
+  Field& xs = L.field(S_XS, xvec, n);
   Field& rs = L.field(S_RS, svec, n);
   Field& ps = L.field(S_PS, sps, n);
   Field& aps = L.field(S_APS, sps, n);
@@ -905,7 +909,7 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nrhs, cg.s.rr_upd, RED_CG_INIT, cg.s, L.stream));
   CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), svec, rs.ref(), n, L.stream));
   CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), sps, ps.ref(), n, L.stream));
-  CPSHIP_CHECK(blas_zero(svec, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
+  CPSHIP_CHECK(blas_zero(xvec, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
   {
     // the first iteration's direction update p = r + 0 p
     std::vector<double> zeros(nrhs, 0.0);
@@ -949,7 +953,7 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
       int e = apply_mdagm(L, op, hprec, apsw, psw, t1s.ref(), t2s.ref(), part, nw);
       if (e) return e;
       CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
-      CPSHIP_CHECK(blas_cg_xr_mixed(svec, xsw, sps, psw, svec, rsw, sps, apsw, sw, o.per_rhs, nw, bsb, part,
+      CPSHIP_CHECK(blas_cg_xr_mixed(xvec, xsw, sps, psw, svec, rsw, sps, apsw, sw, o.per_rhs, nw, bsb, part,
                                     L.stream));
       CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, nullptr, RED_CG_BETA, sw, L.stream));
       return 0;
@@ -976,8 +980,8 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     if (since >= every || paused || done >= max_itr) {
       // ---- FP64-critical steps: accumulate the solution, recompute r ----
       // (on the window: the rhs outside it have converged and keep x)
-      CPSHIP_CHECK(blas_axpy(svec, xsw, STORE_D, xw, nullptr, 1.0, o.per_rhs, nw, bsb, L.stream));
-      CPSHIP_CHECK(blas_zero(svec, xsw, o.per_rhs, nw, bsb, L.stream));
+      CPSHIP_CHECK(blas_axpy(xvec, xsw, STORE_D, xw, nullptr, 1.0, o.per_rhs, nw, bsb, L.stream));
+      CPSHIP_CHECK(blas_zero(xvec, xsw, o.per_rhs, nw, bsb, L.stream));
       rc = apply_mdagm(L, op, PREC_D_D18, apw, xw, t1.ref(), t2.ref(), nullptr, nw);
       if (rc) return rc;
       CPSHIP_CHECK(blas_sub_norm_d(srcw, apw, rw, o.per_rhs, nw, bsb, part, L.stream));

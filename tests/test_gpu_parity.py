@@ -136,6 +136,24 @@ def test_dwf_mixed_cg(oracle, dwf4, prec):
     assert rel(got, ref) < 1e-8
 
 
+@pytest.mark.parametrize("vectors", ["xhalf", "half"])
+def test_dwf_mixed_cg_half_sloppy_vectors(oracle, monkeypatch, dwf4, vectors):
+    """The sloppy solution (and with "half" also the sloppy residual) stored
+    in FP16 with per-site norms: the FP64 residual still meets the stopping
+    rule and the solution matches the FP64 oracle."""
+    monkeypatch.setenv("CPS_HIP_SLOPPY_VECTORS", vectors)
+    lat, u, L, ls = dwf4
+    d = oracle.dwf_struct(L, ls, 1.8)
+    src = F.gaussian_fermion(lat.vcb * ls, 46)
+    ref, it_ref, _ = oracle.dwf_invcg(u, src, 0.01, 1e-12, 5000, d)
+    op = C.make_op(C.OP_DWF, mass=0.01, m5=1.8)
+    got, res = lat.invcg(op, src, C.make_cg_arg(1e-10, 5000, precision=C.PREC_MIXED_HALF))
+    print("dwf mixed sloppy vectors", vectors, it_ref, res[0].iterations, res[0].true_res, flush=True)
+    assert res[0].status == 1
+    assert res[0].true_res <= 1e-10
+    assert rel(got, ref) < 1e-8
+
+
 def test_dwf_multi_rhs_matches_single(dwf4):
     lat, u, L, ls = dwf4
     op = C.make_op(C.OP_DWF, mass=0.01, m5=1.8)
