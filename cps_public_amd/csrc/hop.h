@@ -30,7 +30,12 @@ enum HopFlags {
                           // the 8 links of each of its sites in LDS once (f16
                           // words, third row rebuilt), and every thread keeps
                           // all 8 neighbour spinors in flight
+  HOP_RHSGROUP = 32,      // Wilson kind, nb a power of two, no partials: a
+                          // wavefront computes S sites x G rhs (x nb slots),
+                          // G = 2 << ((flags >> 6) & 3), so the G lanes of a
+                          // site share its link and clover loads
 };
+inline int hop_rhsgroup_G(int flags) { return (flags & HOP_RHSGROUP) ? (2 << ((flags >> 6) & 3)) : 1; }
 
 struct HopParams {
   int flags;         // HopFlags
@@ -74,6 +79,8 @@ struct HopParams {
   // HOP_LDS_LINKS: sites per block the LDS area is sized for (set by the
   // launcher)
   int lds_sites;
+  // HOP_RHSGROUP: log2 of the rhs per wavefront (set by the launcher)
+  int rg_shift;
   int clov;
   int rhs_per_set;
   int rhs0;          // clover set of local rhs r: (rhs0 + r) / rhs_per_set
@@ -108,6 +115,18 @@ constexpr int LDS_SITE_WORDS = 68;
 // sites of a block of bs consecutive elements, nb elements per site (at most)
 inline int lds_sites_per_block(int bs, int nb) { return bs % nb == 0 ? bs / nb : (bs - 1) / nb + 2; }
 inline size_t lds_link_bytes(int sites) { return (size_t)sites * (LDS_SITE_WORDS + 8) * 4; }
+
+// HOP_RHSGROUP: whether the launch shape allows it at block size bs; sets
+// the rhs-per-wavefront shift
+inline bool rhsgroup_setup(HopParams& Q, int bs) {
+  const int G = hop_rhsgroup_G(Q.flags);
+  if (Q.kind != KIND_WILSON || Q.partial || (Q.flags & (HOP_SITEWAVE | HOP_LDS_LINKS | HOP_SPATIAL))) return false;
+  if (Q.nb < 1 || (Q.nb & (Q.nb - 1)) || G * Q.nb > 64 || Q.nrhs % G || bs < 64 || bs % 64) return false;
+  const int S = 64 / (G * Q.nb);
+  if (Q.nsites_out % ((bs / 64) * S)) return false;
+  Q.rg_shift = __builtin_ctz((unsigned)G);
+  return true;
+}
 
 // block size dividing n (a power of two, at most 256)
 inline int pick_block(long long n) {

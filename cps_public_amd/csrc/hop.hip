@@ -26,6 +26,7 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
       return hipErrorInvalidValue;
     shm = lds_link_bytes(Q.lds_sites);
   }
+  if ((P.flags & HOP_RHSGROUP) && !rhsgroup_setup(Q, bs)) return hipErrorInvalidValue;
 #define CPSHIP_HOP_LAUNCH(DAG, KIND, SW)                                                                      \
   if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, true, CLOV_NONE, SW>), grid, block, shm, st, Q); \
   else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_NONE, SW>), grid, block, shm, st, Q);
@@ -42,6 +43,8 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_DWF, 0) } else { CPSHIP_HOP_LAUNCH(0, KIND_DWF, 0) }
   } else if (P.flags & HOP_SITEWAVE) {
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_WILSON, 1) } else { CPSHIP_HOP_LAUNCH(0, KIND_WILSON, 1) }
+  } else if (P.flags & HOP_RHSGROUP) {
+    if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_WILSON, 3) } else { CPSHIP_HOP_LAUNCH(0, KIND_WILSON, 3) }
   } else {
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_WILSON, 0) } else { CPSHIP_HOP_LAUNCH(0, KIND_WILSON, 0) }
   }

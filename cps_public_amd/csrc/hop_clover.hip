@@ -38,11 +38,13 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
       return hipErrorInvalidValue;
     shm = lds_link_bytes(Q.lds_sites);
   }
+  if ((P.flags & HOP_RHSGROUP) && !rhsgroup_setup(Q, bs)) return hipErrorInvalidValue;
 #define CPSHIP_HOP_LAUNCH2(DAG, CLOV, SW)                                                                        \
   if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, true, CLOV, SW>), grid, block, shm, st, Q); \
   else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, false, CLOV, SW>), grid, block, shm, st, Q);
 #define CPSHIP_HOP_LAUNCH(DAG, CLOV)                                                              \
   if (P.flags & HOP_SITEWAVE) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 1) }                                \
+  else if (P.flags & HOP_RHSGROUP) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 3) }                           \
   else if (lds) {                                                                                 \
     if constexpr (std::is_same<SIN, SpinH>::value && CLOV == CLOV_S5X) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 2) } \
     else return hipErrorInvalidValue;                                                             \

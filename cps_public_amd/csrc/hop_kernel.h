@@ -614,6 +614,27 @@ __device__ __forceinline__ void hop_block_sites(const HopParams& P, int el0, int
   E.sl = q - E.qb;
 }
 
+// HOP_RHSGROUP decode (Wilson kind, nb a power of two, no partials): lane =
+// (g S + q) nb + j with g < G rhs and q < S = 64 / (G nb) consecutive sites;
+// logical block sb = (rhs group, block of W S sites), W wavefronts per block
+__device__ __forceinline__ HopElem hop_decode_rhsgroup(const HopParams& P, int64_t sb64) {
+  HopElem E;
+  const int sb = (int)sb64;
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6), W = (int)(blockDim.x >> 6);
+  const int s_shift = 6 - P.rg_shift - P.nb_shift;   // log2 S
+  const int per_g = P.nsites_out / (W << s_shift);
+  const int rg = sb / per_g;
+  const int sblk = sb - rg * per_g;
+  const int gq = lane >> P.nb_shift;
+  const int g = gq >> s_shift;
+  E.j = lane & (P.nb - 1);
+  E.rhs = (rg << P.rg_shift) + g;
+  E.site = ((sblk * W + w) << s_shift) + (gq - (g << s_shift));
+  E.pslot = 0;   // no partials in this mode
+  E.qb = E.nsl = E.sl = E.b = 0;
+  return E;
+}
+
 // Block-uniform quantities (rhs, sub-block) are computed with 32-bit
 // arithmetic on the block index (scalar registers); per thread only the
 // element within the sub-block is split into (site, j), with a shift when
@@ -900,13 +921,15 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
 // SW = 0: one output element per thread in element order; SW = 1
 // (HOP_SITEWAVE): one site per wavefront, so the site's neighbour entries
 // and links are read with scalar loads; SW = 2 (HOP_LDS_LINKS): element
-// order, links staged per block in LDS.
+// order, links staged per block in LDS; SW = 3 (HOP_RHSGROUP): S sites x G
+// rhs per wavefront.
 template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV, int SW>
 __global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) {
   extern __shared__ uint4 hop_lds_area[];   // HOP_LDS_LINKS (SW = 2): the block's links
   const int64_t nsub = (int64_t)gridDim.x;
   const int64_t hb = logical_block(blockIdx.x, gridDim.x, !(P.flags & HOP_NO_XCD_REMAP));
-  const HopElem E = SW == 1 ? hop_decode_sitewave(P, hb, nsub) : hop_decode<KIND>(P, hb, nsub);
+  const HopElem E = SW == 1 ? hop_decode_sitewave(P, hb, nsub)
+                   : SW == 3 ? hop_decode_rhsgroup(P, hb) : hop_decode<KIND>(P, hb, nsub);
   const Nbr N = hop_nbr<KIND>(P, E);
   const R nrm = hop_element<SIN, SOUT, LK, R, DAG, KIND, NT, CLOV, SW == 2>(
       P, E, N, reinterpret_cast<uint32_t*>(hop_lds_area));

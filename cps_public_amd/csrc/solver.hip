@@ -325,6 +325,13 @@ static bool sitewave_ok(Lattice& L, const HopParams& P, int b, bool spatial) {
   return true;
 }
 
+// whether a launch may use HOP_RHSGROUP with these flags at block size b
+static bool rhsgroup_ok(const HopParams& P, int b, int flags) {
+  HopParams Q = P;
+  Q.flags = flags;
+  return rhsgroup_setup(Q, b);
+}
+
 // whether a launch may use HOP_LDS_LINKS at block size b (half-precision
 // spinors, at most one link per thread to stage)
 static bool lds_links_ok(const HopParams& P, int b) {
@@ -354,7 +361,7 @@ int tuned_launch(Lattice& L, HopParams& P) {
     P.perm = (const uint32_t*)L.perm.p;
   int mode = 0;
   if (const char* m = getenv("CPS_HIP_HOP_MODE"))
-    mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3 : 0;
+    mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3 : !strcmp(m, "rhsgroup") ? 4 : 0;
   HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr,
                  P.clov, {L.g.L[0], L.g.L[1], L.g.L[2], L.g.L[3]}, mode};
   // the winners are kept per process (every handle, engine and thread of the
@@ -381,6 +388,8 @@ int tuned_launch(Lattice& L, HopParams& P) {
         chosen = kv.second;
         if ((chosen.flags & HOP_SITEWAVE) && !sitewave_ok(L, P, chosen.block, (chosen.flags & HOP_SPATIAL) != 0))
           chosen.flags &= ~HOP_SITEWAVE;
+        if ((chosen.flags & HOP_RHSGROUP) && !rhsgroup_ok(P, chosen.block, chosen.flags))
+          chosen.flags &= ~(HOP_RHSGROUP | (3 << 6));
         have = true;
         tune_cache.emplace(key, chosen);
         break;
@@ -401,8 +410,19 @@ int tuned_launch(Lattice& L, HopParams& P) {
       CPSHIP_CHECK(hipEventCreate(&e0));
       CPSHIP_CHECK(hipEventCreate(&e1));
       float best_ms = 1e30f;
+      // candidate flags: every combination of the first five variants, then
+      // the rhs-group mapping for G = 2..16 with and without streaming stores
+      std::vector<int> cand;
+      for (int f = 0; f < 32; f++) cand.push_back(f);
+      for (int g = 0; g < 4; g++)
+        for (int nt = 0; nt < 2; nt++) cand.push_back(HOP_RHSGROUP | (g << 6) | nt);
       for (int b : blocks)
-        for (int f = 0; f < 32; f++) {
+        for (int f : cand) {
+          if (f & HOP_RHSGROUP) {
+            if (!rhsgroup_ok(P, b, f) || mode == 1 || mode == 2 || mode == 3) continue;
+          } else if (mode == 4 && rhsgroup_ok(P, b, HOP_RHSGROUP)) {
+            continue;   // CPS_HIP_HOP_MODE=rhsgroup
+          }
           if (f & HOP_SPATIAL) {
             if ((f & HOP_NO_XCD_REMAP) || !spatial_ok(L, P, b)) continue;
           }

@@ -743,6 +743,36 @@ def test_sitewave_mapping_bit_identical(oracle, monkeypatch, prec):
             assert rel(res["sitewave"][0][k * n:(k + 1) * n], ref) < 1e-13
 
 
+@pytest.mark.parametrize("case", ["clover", "mobius", "wilson"])
+def test_rhsgroup_mapping_bit_identical(monkeypatch, case):
+    """Operators whose hops run with several rhs of one site per wavefront
+    (HOP_RHSGROUP; clover with two kappas, rhs r on kappa r // 2) give the
+    same bits as the element mapping, FP64 and half precision."""
+    L = [4, 4, 4, 8]
+    u = wilson_gauge(L, 0.2, 250)
+    ls = 8 if case == "mobius" else 1
+    nrhs = 8 if case == "wilson" else 4
+    vcb = int(np.prod(L)) // 2
+    src = np.concatenate([F.gaussian_fermion(vcb * ls, 251 + k) for k in range(nrhs)])
+    res = {}
+    for mode in ("element", "rhsgroup"):
+        monkeypatch.setenv("CPS_HIP_HOP_MODE", mode)
+        lat = C.Lattice(L, ls, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        if case == "clover":
+            lat.clover_build(CLOVER_KAPPAS, CLOVER_CSW)
+            op = clover_op(CLOVER_KAPPAS[0])
+        elif case == "mobius":
+            op = C.make_op(C.OP_MOBIUS, mass=0.05, m5=1.8, b=1.5, c=0.5)
+        else:
+            op = C.make_op(C.OP_WILSON, mass=0.1)
+        res[mode] = (lat.matpc(op, src, 0, nrhs=nrhs), lat.matpc(op, src, 1, nrhs=nrhs),
+                     lat.mdagm_prec(op, src, C.PREC_MIXED_HALF, nrhs=nrhs)[0])
+        lat.close()
+    for a, b_ in zip(res["element"], res["rhsgroup"]):
+        assert np.array_equal(a, b_)
+
+
 @pytest.mark.parametrize("kind,ls,nrhs", [("mobius", 16, 2), ("mobius", 12, 1), ("mobius", 8, 1), ("dwf", 16, 2)])
 def test_lds_links_mapping_bit_identical(monkeypatch, kind, ls, nrhs):
     """Half-precision MdagM (both hops, the first with its |M x|^2 partial
