@@ -52,10 +52,10 @@ CONFIGS = {
     # BF16's 8-bit mantissa costs 2.5x the CG iterations at this mass)
     "c3h": dict(name="mobius_dwf_24^3x64_ls16_b1.5_c0.5_mf0.005_12rhs_fp16links", L=[24, 24, 24, 64], ls=16,
                 kind=C.OP_MOBIUS, eps=0.2, mass=0.005, m5=1.8, b=1.5, c=0.5, nrhs=12, src="point",
-                mixed=C.PREC_MIXED_HALF, seed=303),
+                mixed=C.PREC_MIXED_HALF, seed=303, reliable_delta=0.01),
     "c4": dict(name="clover_32^3x64_csw1.0_kappa0.1360-0.1375_48rhs", L=[32, 32, 32, 64], ls=1,
                kind=C.OP_CLOVER, eps=0.15, kappas=[0.1360, 0.1365, 0.1370, 0.1375], csw=1.0, nrhs=48,
-               src="point", mixed=C.PREC_MIXED_HALF, seed=404),
+               src="point", mixed=C.PREC_MIXED_HALF, seed=404, reliable_delta=0.01),
 }
 
 
@@ -260,8 +260,9 @@ def main():
     ap.add_argument("--hop-reps", type=int, default=50)
     ap.add_argument("--max-iter", type=int, default=20000,
                     help="CG iteration cap (a small cap gives a bounded run for kernel traces)")
-    ap.add_argument("--reliable-delta", type=float, default=0.1,
-                    help="mixed CG: reliable update when |r_sloppy| < delta |r| of the last update")
+    ap.add_argument("--reliable-delta", type=float, default=None,
+                    help="mixed CG: reliable update when |r_sloppy| < delta |r| of the last update (default: the "
+                         "workload's value, 0.01 for c3h and c4 and 0.1 otherwise)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only launch the hopping kernels (mixed and FP64) and the reference copy kernel, "
                          "for hardware-counter runs")
@@ -296,8 +297,9 @@ def main():
         lat.close()
         return
     src_dev = C.DeviceBuffer.from_array(src)
+    delta = a.reliable_delta if a.reliable_delta is not None else cfg.get("reliable_delta", 0.1)
     arg_main = C.make_cg_arg(1e-10, a.max_iter, precision=prec_main, reliable_every=100,
-                             reliable_delta=a.reliable_delta)
+                             reliable_delta=delta)
     t_main, res_main = time_solves(lat, op, src_dev, arg_main, nrhs, a.warmup, a.steps)
     ms = 1e3 * float(np.mean(t_main))
     s_per_rhs = float(np.mean(t_main)) / nrhs
@@ -343,6 +345,7 @@ def main():
         "dtype": "f32" if mixed is not None else "f64",
         "data": "synthetic (seeded SU(3) links exp(i eps H), Philox)",
         "config": {"workload": cfg["name"], "nrhs": nrhs, "stop_rsd": 1e-10,
+                   "reliable_delta": delta if mixed is not None else None, "reliable_every": 100,
                    "precision": {0: "fp64", 1: "mixed_fp16", 2: "mixed_bf16links", 3: "mixed_fp32"}[prec_main],
                    "storage": {0: "f64 spinors and links",
                                1: "f16 spinors (per-site norms) + f16 links, f32 CG vectors, f64 reliable updates",
