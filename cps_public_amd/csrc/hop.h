@@ -81,6 +81,14 @@ struct HopParams {
   int lds_sites;
   // HOP_RHSGROUP: log2 of the rhs per wavefront (set by the launcher)
   int rg_shift;
+  // m5post (half-precision Wilson-kind hop with nb = Ls = 16, no epilogue
+  // field): the Moebius fifth-dimension stage of the hop's dag applied to
+  // out = scale*hop in registers, the 16 slots of a site being one 16-lane
+  // row of the wavefront: dag 0 out = b5 M5^-1 h + c5 P M5^-1 h, dag 1
+  // out = M5^-dag (b5 h + c5 P^dag h); M5 with ratio K = -m5a and the walls'
+  // mass m5mass, m5inv_d = 1 / (1 + m5mass m5a^16)
+  int m5post;
+  double m5a, m5inv_d, m5mass, m5b, m5c;
   int clov;
   int rhs_per_set;
   int rhs0;          // clover set of local rhs r: (rhs0 + r) / rhs_per_set

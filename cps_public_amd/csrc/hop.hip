@@ -27,6 +27,9 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
     shm = lds_link_bytes(Q.lds_sites);
   }
   if ((P.flags & HOP_RHSGROUP) && !rhsgroup_setup(Q, bs)) return hipErrorInvalidValue;
+  // the fused fifth-dimension stage needs the 16 slots of a site in one row
+  if (P.m5post && (!std::is_same<SIN, SpinH>::value || P.kind != KIND_WILSON || P.nb != 16 || P.X.data || bs % 16))
+    return hipErrorInvalidValue;
 #define CPSHIP_HOP_LAUNCH(DAG, KIND, SW)                                                                      \
   if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, true, CLOV_NONE, SW>), grid, block, shm, st, Q); \
   else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_NONE, SW>), grid, block, shm, st, Q);

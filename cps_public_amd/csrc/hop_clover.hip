@@ -39,6 +39,7 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
     shm = lds_link_bytes(Q.lds_sites);
   }
   if ((P.flags & HOP_RHSGROUP) && !rhsgroup_setup(Q, bs)) return hipErrorInvalidValue;
+  if (P.m5post) return hipErrorInvalidValue;   // plain hops only
 #define CPSHIP_HOP_LAUNCH2(DAG, CLOV, SW)                                                                        \
   if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, true, CLOV, SW>), grid, block, shm, st, Q); \
   else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, false, CLOV, SW>), grid, block, shm, st, Q);

@@ -738,6 +738,102 @@ __device__ __forceinline__ Nbr hop_nbr(const HopParams& P, const HopElem& E) {
   return N;
 }
 
+// ---- Moebius fifth-dimension stage across a 16-lane row -----------------
+// With nb = Ls = 16 the slots s = 0..15 of a site are lanes 16r .. 16r+15 of
+// the wavefront (every thread mapping keeps them so), and the M5^-1
+// recurrence (m5inv_regs) becomes log-step scans over the row: DPP row
+// shifts (slots from outside the row read 0), rotations and the broadcast of
+// the row's lane 15.
+enum { DPP_ROW_SHL = 0x100, DPP_ROW_SHR = 0x110, DPP_ROW_ROR = 0x120, DPP_ROW_NEWBCAST = 0x150 };
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+
+// forward type: y_s = sum_{s' <= s} a^(s-s') x_s' (scan), then
+// y_s += fF_s y_15 (s < 15), y_15 *= inv_d
+__device__ __forceinline__ float m5_fwd(float x, bool last, const float (&ap)[4], float fF, float inv_d) {
+  float y = x;
+  y = fmaf(ap[0], dpp_f<DPP_ROW_SHR + 1>(y), y);
+  y = fmaf(ap[1], dpp_f<DPP_ROW_SHR + 2>(y), y);
+  y = fmaf(ap[2], dpp_f<DPP_ROW_SHR + 4>(y), y);
+  y = fmaf(ap[3], dpp_f<DPP_ROW_SHR + 8>(y), y);
+  const float y15 = dpp_f<DPP_ROW_NEWBCAST + 15>(y);
+  return last ? y * inv_d : fmaf(fF, y15, y);
+}
+
+// backward type: x_15 <- inv_d x_15 + sum_{s < 15} fB_s x_s, then the
+// suffix scan y_s = sum_{s' >= s} a^(s'-s) x_s'
+__device__ __forceinline__ float m5_bwd(float x, bool last, const float (&ap)[4], float fB, float inv_d) {
+  float z = fB * x;   // fB_15 = 0
+  z += dpp_f<DPP_ROW_SHR + 1>(z);
+  z += dpp_f<DPP_ROW_SHR + 2>(z);
+  z += dpp_f<DPP_ROW_SHR + 4>(z);
+  z += dpp_f<DPP_ROW_SHR + 8>(z);
+  float y = last ? fmaf(inv_d, x, z) : x;
+  y = fmaf(ap[0], dpp_f<DPP_ROW_SHL + 1>(y), y);
+  y = fmaf(ap[1], dpp_f<DPP_ROW_SHL + 2>(y), y);
+  y = fmaf(ap[2], dpp_f<DPP_ROW_SHL + 4>(y), y);
+  y = fmaf(ap[3], dpp_f<DPP_ROW_SHL + 8>(y), y);
+  return y;
+}
+
+// b5 v + c5 (P v) for one real of the upper (UP) or lower spins; FROM_M: the
+// shifted term comes from slot s-1 (else s+1), -mass at the wall
+template <bool FROM_M>
+__device__ __forceinline__ float s5_row(float v, float b5, float cw) {
+  const float sh = FROM_M ? dpp_f<DPP_ROW_ROR + 1>(v) : dpp_f<DPP_ROW_ROR + 15>(v);
+  return fmaf(cw, sh, b5 * v);
+}
+
+template <int DAG>
+__device__ __forceinline__ void m5_row(const HopParams& P, int s, cplx<float> (&res)[12]) {
+  const float a = float(P.m5a), inv_d = float(P.m5inv_d), mass = float(P.m5mass);
+  const float b5 = float(P.m5b), c5 = float(P.m5c);
+  float ap[4];
+  ap[0] = a;
+  ap[1] = a * a;
+  ap[2] = ap[1] * ap[1];
+  ap[3] = ap[2] * ap[2];
+  float as = 1.f;   // a^s
+  if (s & 1) as *= ap[0];
+  if (s & 2) as *= ap[1];
+  if (s & 4) as *= ap[2];
+  if (s & 8) as *= ap[3];
+  const bool last = s == 15;
+  const float fF = last ? 0.f : -mass * inv_d * a * as;   // - mass inv_d a^(s+1)
+  const float fB = last ? 0.f : -a * mass * inv_d * as;   // K mass inv_d a^s
+  const float cm = c5 * (s == 0 ? -mass : 1.f), cp = c5 * (last ? -mass : 1.f);
+  if constexpr (DAG == 0) {
+    // M5^-1: upper spins forward type, lower backward type; then the s-term
+    // with P of dag 0 (upper from s-1, lower from s+1)
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      res[k].re = s5_row<true>(m5_fwd(res[k].re, last, ap, fF, inv_d), b5, cm);
+      res[k].im = s5_row<true>(m5_fwd(res[k].im, last, ap, fF, inv_d), b5, cm);
+    }
+#pragma unroll
+    for (int k = 6; k < 12; k++) {
+      res[k].re = s5_row<false>(m5_bwd(res[k].re, last, ap, fB, inv_d), b5, cp);
+      res[k].im = s5_row<false>(m5_bwd(res[k].im, last, ap, fB, inv_d), b5, cp);
+    }
+  } else {
+    // the s-term with P of dag 1 (upper from s+1, lower from s-1), then
+    // M5^-dag: upper spins backward type, lower forward type
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      res[k].re = m5_bwd(s5_row<false>(res[k].re, b5, cp), last, ap, fB, inv_d);
+      res[k].im = m5_bwd(s5_row<false>(res[k].im, b5, cp), last, ap, fB, inv_d);
+    }
+#pragma unroll
+    for (int k = 6; k < 12; k++) {
+      res[k].re = m5_fwd(s5_row<true>(res[k].re, b5, cm), last, ap, fF, inv_d);
+      res[k].im = m5_fwd(s5_row<true>(res[k].im, b5, cm), last, ap, fF, inv_d);
+    }
+  }
+}
+
 // One output element: the hop, the fifth-dimension term (DWF), the epilogue
 // and the store; returns the element's |out|^2 when partials are requested.
 template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV, int LDSL>
@@ -906,6 +1002,9 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
       } else {
 #pragma unroll
         for (int k = 0; k < 12; k++) res[k] = cmk<R>(sc * acc[k].re, sc * acc[k].im);
+      }
+      if constexpr (std::is_same<SIN, SpinH>::value && KIND == KIND_WILSON) {
+        if (P.m5post) m5_row<DAG>(P, j, res);
       }
     }
     spin_store<SOUT, R, NT>(P.out, eo, res);

@@ -511,6 +511,19 @@ static void mobius_kappas(const cps_hip_op* op, double* kb, double* kc) {
 // 4-D hop of every slice (Wilson kernel, nb = Ls), output parity `target`;
 // with s5x the epilogue adds X + x5coef P X (P of direction dag, -mass walls)
 // and writes the per-block |out|^2 partials when partial is given
+// the Moebius fifth-dimension stage fused into a half-precision hop's
+// epilogue (HopParams::m5post): ratio K = kappa_b / kappa_c, the walls' mass,
+// and the coefficients of b5 + c5 P
+struct M5Post {
+  double K, mass, b, c;
+};
+
+// CPS_HIP_M5_FUSED=0 keeps M5^-1 as its own pass
+static bool m5_fused_enabled() {
+  const char* e = getenv("CPS_HIP_M5_FUSED");
+  return !(e && *e == '0');
+}
+
 // CPS_HIP_S5_FUSED=0 keeps the dag-1 s-term as its own pass
 static bool s5_fused_enabled() {
   const char* e = getenv("CPS_HIP_S5_FUSED");
@@ -520,7 +533,7 @@ static bool s5_fused_enabled() {
 // s5x: out = X + x5coef (P X) + scale (hb5 hop + hc5 P hop) (CLOV_S5X)
 static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag, double scale,
                 int nrhs, bool s5x = false, const FieldRef* X = nullptr, double x5coef = 0.0, double mass = 0.0,
-                double* partial = nullptr, double hb5 = 1.0, double hc5 = 0.0) {
+                double* partial = nullptr, double hb5 = 1.0, double hc5 = 0.0, const M5Post* m5 = nullptr) {
   cps_hip_op w;
   std::memset(&w, 0, sizeof(w));
   w.kind = CPS_HIP_OP_MOBIUS;   // single-parity layout with nb = Ls
@@ -550,6 +563,14 @@ static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, 
     P.hb5 = hb5;
     P.hc5 = hc5;
   }
+  if (m5) {
+    P.m5post = 1;
+    P.m5a = -m5->K;
+    P.m5inv_d = 1.0 / (1.0 + m5->mass * std::pow(m5->K, 16));
+    P.m5mass = m5->mass;
+    P.m5b = m5->b;
+    P.m5c = m5->c;
+  }
   const int lstore = link_storage_of(hprec);
   int rc = L.ensure_gauge(lstore);
   if (rc) return rc;
@@ -577,11 +598,21 @@ int mobius_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, c
   const double ratio = kb / kc;
   const double b = op->mobius_b, c = op->mobius_c;
   int rc;
+  // half precision with Ls = 16: M5^-1 and its s-term run in the epilogue of
+  // the first hop (the 16 slots of a site are one 16-lane row)
+  const bool m5f = store_of(hprec) == STORE_H && L.ls == 16 && m5_fused_enabled();
+  const M5Post m5{ratio, op->mass, b, c};
   if (dag == 0) {
     // t1 = b in + c P in ; out = kb * D4(cb=1) t1
     S5Params P = s5_base(L, op, 0, nrhs);
     P.out = t1; P.A = in; P.a1 = 1; P.b1 = b; P.c1 = c;
     if ((rc = launch_s5(L, hprec, P))) return rc;
+    if (m5f) {
+      // t2 = b M5^-1 (kb D4 t1) + c P M5^-1 (kb D4 t1) ;
+      // out = in + (kb/kc) P in - kb * D4(cb=0) t2
+      if ((rc = hop4(L, hprec, t2, t1, 0, 0, kb, nrhs, false, nullptr, 0.0, 0.0, nullptr, 1.0, 0.0, &m5))) return rc;
+      return hop4(L, hprec, out, t2, 1, 0, -kb, nrhs, true, &in, ratio, op->mass, partial);
+    }
     if ((rc = hop4(L, hprec, out, t1, 0, 0, kb, nrhs))) return rc;
     // t1 = b M5^-1 out + c P M5^-1 out (one kernel)
     if ((rc = launch_m5inv(L, hprec, out, 0, ratio, op->mass, nrhs, &t1, S5_AFTER, b, c))) return rc;
@@ -589,8 +620,12 @@ int mobius_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, c
     return hop4(L, hprec, out, t1, 1, 0, -kb, nrhs, true, &in, ratio, op->mass, partial);
   }
   // dag 1: t1 = -kb * D4dag(cb=1) in ; t2 = M5^-dag (b t1 + c Pdag t1)
-  if ((rc = hop4(L, hprec, t1, in, 0, 1, -kb, nrhs))) return rc;
-  if ((rc = launch_m5inv(L, hprec, t1, 1, ratio, op->mass, nrhs, &t2, S5_BEFORE, b, c))) return rc;
+  if (m5f) {
+    if ((rc = hop4(L, hprec, t2, in, 0, 1, -kb, nrhs, false, nullptr, 0.0, 0.0, nullptr, 1.0, 0.0, &m5))) return rc;
+  } else {
+    if ((rc = hop4(L, hprec, t1, in, 0, 1, -kb, nrhs))) return rc;
+    if ((rc = launch_m5inv(L, hprec, t1, 1, ratio, op->mass, nrhs, &t2, S5_BEFORE, b, c))) return rc;
+  }
   // t1 = kb * D4dag(cb=0) t2 ; out = (b t1 + c Pdag t1) + (in + ratio Pdag in):
   // one hop with the s-term fused into its epilogue when the Ls slots of a
   // site fit a wavefront's lane groups, else a hop and a k_s5 pass

@@ -363,7 +363,7 @@ int tuned_launch(Lattice& L, HopParams& P) {
   if (const char* m = getenv("CPS_HIP_HOP_MODE"))
     mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3 : !strcmp(m, "rhsgroup") ? 4 : 0;
   HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr,
-                 P.clov, {L.g.L[0], L.g.L[1], L.g.L[2], L.g.L[3]}, mode};
+                 P.clov + 16 * P.m5post, {L.g.L[0], L.g.L[1], L.g.L[2], L.g.L[3]}, mode};
   // the winners are kept per process (every handle, engine and thread of the
   // same geometry reuses them); a launch shape is timed once
   static std::mutex tune_mu;

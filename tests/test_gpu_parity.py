@@ -284,6 +284,39 @@ def test_mobius_mdag_s5_fusion(oracle, monkeypatch, ls, fused):
     lat.close()
 
 
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_mobius_m5_fused_in_hop(oracle, monkeypatch, nrhs):
+    """Half precision, Ls = 16: M5^-1 and its s-term run across the 16 lanes
+    of a site in the first hop's epilogue.  The fused M^dag M agrees with
+    FP64 at least as well as the separate M5^-1 pass does (it skips one
+    FP16 rounding of the intermediate), and a mixed solve with it reaches the
+    FP64 stopping rule and the oracle's solution."""
+    L, ls = [4, 4, 4, 8], 16
+    u = wilson_gauge(L, 0.2, 260)
+    op = C.make_op(C.OP_MOBIUS, mass=0.05, m5=1.8, b=1.5, c=0.5)
+    vcb = int(np.prod(L)) // 2
+    src = np.concatenate([F.gaussian_fermion(vcb * ls, 261 + k) for k in range(nrhs)])
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("CPS_HIP_M5_FUSED", fused)
+        lat = C.Lattice(L, ls, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        out[fused] = lat.mdagm_prec(op, src, C.PREC_MIXED_HALF, nrhs=nrhs)[0]
+        d64 = lat.mdagm(op, src, nrhs=nrhs)[0]
+        if fused == "1":
+            x, res = lat.invcg(op, src, C.make_cg_arg(1e-10, 5000, precision=C.PREC_MIXED_HALF), nrhs=nrhs)
+        lat.close()
+    e1, e0 = rel(out["1"], d64), rel(out["0"], d64)
+    print("m5 fused", nrhs, e1, e0, [r.iterations for r in res], flush=True)
+    assert e1 < 5e-3 and e0 < 5e-3
+    assert e1 <= 1.5 * e0
+    assert max(r.true_res for r in res) <= 1e-10
+    m = oracle.mobius_struct(L, ls, 1.8, 1.5, 0.5)
+    n = 24 * vcb * ls
+    ref, _, _ = oracle.mobius_invcg(u, src[:n], 0.05, 1e-12, 5000, m)
+    assert rel(x[:n], ref) < 1e-8
+
+
 def test_mobius_cg(oracle, mob4):
     lat, u, L, ls = mob4
     m = oracle.mobius_struct(L, ls, 1.8, 1.5, 0.5)
