@@ -67,6 +67,10 @@ struct HopParams {
   // P hop term (hc5 != 0) takes the hop of slots s-1, s+1 from the
   // neighbouring lanes, so it needs nb a power of two dividing 64
   double x5coef, x5mass, hb5, hc5;
+  // CLOV_S5X with FP16 fields and nb = 16: the slots s-1, s+1 of X and of the
+  // hop come from the neighbouring lanes of the site's 16-lane row (DPP
+  // rotations) instead of loads and lane permutes
+  int s5_row;
   const uint4* nbr;         // neighbour table (k_build_nbr)
   const uint32_t* nbr_neg;
   // HOP_SPATIAL: site_cb of logical site b*perm_S + q (the same table for
@@ -96,7 +100,9 @@ struct HopParams {
   const double* kscale;
 };
 
-enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2, CLOV_S5X = 3 };
+// CLOV_S5XR: CLOV_S5X with the row exchange of s5_row (a kernel of its own,
+// chosen by the launcher; not a value of HopParams::clov)
+enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2, CLOV_S5X = 3, CLOV_S5XR = 4 };
 
 hipError_t launch_hop(const HopParams& P, hipStream_t st);
 hipError_t build_nbr(const Geom& g, uint4* nbr, uint32_t* nbr_neg, hipStream_t st);

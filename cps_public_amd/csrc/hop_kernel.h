@@ -926,7 +926,7 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
 
     const int64_t eo = (int64_t)rhs * per_rhs + rem;
     cplx<R> res[12];
-    if constexpr (CLOV == CLOV_S5X) {
+    if constexpr (CLOV == CLOV_S5X || CLOV == CLOV_S5XR) {
       // j = s of the Moebius layout; the Ls slots of this site are adjacent
       const int ls = P.nb;
       const int64_t sbase = eo - j;
@@ -940,8 +940,17 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
       constexpr int c_from_p = DAG ? 0 : 6;
       cplx<R> xv[12], hm[6], hp[6];
       spin_load<SOUT, R>(P.X, eo, xv);
-      spin_load_half<SOUT, R>(P.X, sbase + sm, c_from_m, hm);
-      spin_load_half<SOUT, R>(P.X, sbase + sp, c_from_p, hp);
+      constexpr bool ROW = CLOV == CLOV_S5XR && std::is_same<R, float>::value;
+      if constexpr (ROW) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          hm[k] = cmk<R>(dpp_f<DPP_ROW_ROR + 1>(xv[c_from_m + k].re), dpp_f<DPP_ROW_ROR + 1>(xv[c_from_m + k].im));
+          hp[k] = cmk<R>(dpp_f<DPP_ROW_ROR + 15>(xv[c_from_p + k].re), dpp_f<DPP_ROW_ROR + 15>(xv[c_from_p + k].im));
+        }
+      } else {
+        spin_load_half<SOUT, R>(P.X, sbase + sm, c_from_m, hm);
+        spin_load_half<SOUT, R>(P.X, sbase + sp, c_from_p, hp);
+      }
       const R xc = R(P.xcoef), x5 = R(P.x5coef), sc = R(P.scale);
       const R sb = R(P.scale * P.hb5);
 #pragma unroll
@@ -962,8 +971,18 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
         const R scm = R(P.scale * P.hc5) * cm, scp = R(P.scale * P.hc5) * cp;
 #pragma unroll
         for (int k = 0; k < 6; k++) {
-          const R mre = __shfl(acc[c_from_m + k].re, lm, 64), mim = __shfl(acc[c_from_m + k].im, lm, 64);
-          const R pre = __shfl(acc[c_from_p + k].re, lp, 64), pim = __shfl(acc[c_from_p + k].im, lp, 64);
+          R mre, mim, pre, pim;
+          if constexpr (ROW) {
+            mre = dpp_f<DPP_ROW_ROR + 1>(acc[c_from_m + k].re);
+            mim = dpp_f<DPP_ROW_ROR + 1>(acc[c_from_m + k].im);
+            pre = dpp_f<DPP_ROW_ROR + 15>(acc[c_from_p + k].re);
+            pim = dpp_f<DPP_ROW_ROR + 15>(acc[c_from_p + k].im);
+          } else {
+            mre = __shfl(acc[c_from_m + k].re, lm, 64);
+            mim = __shfl(acc[c_from_m + k].im, lm, 64);
+            pre = __shfl(acc[c_from_p + k].re, lp, 64);
+            pim = __shfl(acc[c_from_p + k].im, lp, 64);
+          }
           res[c_from_m + k].re += scm * mre;
           res[c_from_m + k].im += scm * mim;
           res[c_from_p + k].re += scp * pre;

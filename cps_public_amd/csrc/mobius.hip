@@ -562,6 +562,9 @@ static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, 
     P.x5mass = mass;
     P.hb5 = hb5;
     P.hc5 = hc5;
+    // CPS_HIP_S5X_ROW=0 loads the slots s-1, s+1 of X instead
+    const char* e = getenv("CPS_HIP_S5X_ROW");
+    P.s5_row = (store_of(hprec) == STORE_H && L.ls == 16 && !(e && *e == '0')) ? 1 : 0;
   }
   if (m5) {
     P.m5post = 1;

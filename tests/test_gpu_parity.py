@@ -317,6 +317,26 @@ def test_mobius_m5_fused_in_hop(oracle, monkeypatch, nrhs):
     assert rel(x[:n], ref) < 1e-8
 
 
+def test_mobius_s5_row_exchange_bit_identical(monkeypatch):
+    """Half precision, Ls = 16: the s-term epilogue taking the slots s-1, s+1
+    of X and of the hop from the neighbouring lanes (DPP row rotations) gives
+    the same bits as loading them and exchanging through lane permutes."""
+    L, ls, nrhs = [4, 4, 4, 8], 16, 2
+    u = wilson_gauge(L, 0.2, 270)
+    op = C.make_op(C.OP_MOBIUS, mass=0.05, m5=1.8, b=1.5, c=0.5)
+    vcb = int(np.prod(L)) // 2
+    src = np.concatenate([F.gaussian_fermion(vcb * ls, 271 + k) for k in range(nrhs)])
+    out = {}
+    for row in ("1", "0"):
+        monkeypatch.setenv("CPS_HIP_S5X_ROW", row)
+        lat = C.Lattice(L, ls, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        out[row] = lat.mdagm_prec(op, src, C.PREC_MIXED_HALF, nrhs=nrhs)
+        lat.close()
+    assert np.array_equal(out["1"][0], out["0"][0])
+    assert np.array_equal(out["1"][1], out["0"][1])
+
+
 def test_mobius_cg(oracle, mob4):
     lat, u, L, ls = mob4
     m = oracle.mobius_struct(L, ls, 1.8, 1.5, 0.5)
