@@ -41,6 +41,13 @@ hipError_t reduce_rhs(const double* partial, int bpr, int nrhs, double* out, int
 hipError_t blas_cg_xr_mixed(int sx, const FieldRef& x, int sp, const FieldRef& p, int sr, const FieldRef& r, int sa,
                             const FieldRef& ap, const CGScalars& cg, int64_t per_rhs, int nrhs, int bs,
                             double* partial, hipStream_t s);
+// the mixed CG with a lagged solution step: x += alpha p; p = r + beta p
+// (active rhs), and r -= alpha A p with |r|^2
+hipError_t blas_xpay_x_mixed(int sp, const FieldRef& p, int sr, const FieldRef& r, int sx, const FieldRef& x,
+                             const double* alpha, const double* beta, const int* active, int64_t per_rhs, int nrhs,
+                             int bs, hipStream_t s);
+hipError_t blas_cg_r_mixed(int sr, const FieldRef& r, int sa, const FieldRef& ap, const CGScalars& cg,
+                           int64_t per_rhs, int nrhs, int bs, double* partial, hipStream_t s);
 hipError_t blas_xpay_mixed(int sp, const FieldRef& p, int sr, const FieldRef& r, const double* beta, const int* active,
                            int64_t per_rhs, int nrhs, int bs, hipStream_t s);
 hipError_t blas_zero(int st, const FieldRef& x, int64_t per_rhs, int nrhs, int bs, hipStream_t s);

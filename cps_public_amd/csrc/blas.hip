@@ -148,6 +148,7 @@ __global__ void __launch_bounds__(256) k_xpay(FieldRef p, FieldRef r, const doub
 template <typename SX, typename SY>
 __global__ void __launch_bounds__(256) k_axpy(FieldRef x, FieldRef y, const double* a, double a_const, int64_t per_rhs) {
   const int rhs = blockIdx.y;
+  if (a && a[rhs] == 0.0) return;   // a per-rhs coefficient of 0 leaves y as it is
   const int64_t e = (int64_t)rhs * per_rhs + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   using RX = typename SpinTraits<SX>::R;
   using RY = typename SpinTraits<SY>::R;
@@ -186,6 +187,66 @@ __global__ void __launch_bounds__(256) k_cg_xr_t(FieldRef x, FieldRef p, FieldRe
       n += double(vr[c].re) * vr[c].re + double(vr[c].im) * vr[c].im;
     }
     spin_store<SX, typename SpinTraits<SX>::R>(x, e, vx);
+    spin_store<SR, typename SpinTraits<SR>::R>(r, e, vr);
+  }
+  block_partial_d(n, partial, slot);
+}
+
+// The mixed CG's lagged solution update: the step x += alpha p of the
+// previous iteration is applied by the kernel that reads p next, the
+// direction update p = r + beta p (alpha[rhs] = 0 when no step is pending;
+// a rhs that stopped still gets its last step, but no new direction).
+template <typename SP, typename SR, typename SX>
+__global__ void __launch_bounds__(256) k_xpay_x_t(FieldRef p, FieldRef r, FieldRef x, const double* alpha,
+                                                  const double* beta, const int* active, int64_t per_rhs) {
+  const int rhs = blockIdx.y;
+  const double a = alpha[rhs];
+  const bool act = active[rhs] != 0;
+  if (!act && a == 0.0) return;
+  const int64_t e = (int64_t)rhs * per_rhs + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  using R = float;
+  cplx<typename SpinTraits<SP>::R> vp[12];
+  spin_load<SP, typename SpinTraits<SP>::R>(p, e, vp);
+  if (a != 0.0) {
+    const R ar = R(a);
+    cplx<typename SpinTraits<SX>::R> vx[12];
+    spin_load<SX, typename SpinTraits<SX>::R>(x, e, vx);
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      vx[c].re += ar * R(vp[c].re);
+      vx[c].im += ar * R(vp[c].im);
+    }
+    spin_store<SX, typename SpinTraits<SX>::R>(x, e, vx);
+  }
+  if (act) {
+    const R b = R(beta[rhs]);
+    cplx<typename SpinTraits<SR>::R> vr[12];
+    spin_load<SR, typename SpinTraits<SR>::R>(r, e, vr);
+    cplx<R> o[12];
+#pragma unroll
+    for (int c = 0; c < 12; c++) o[c] = cmk<R>(b * R(vp[c].re) + R(vr[c].re), b * R(vp[c].im) + R(vr[c].im));
+    spin_store<SP, R>(p, e, o);
+  }
+}
+
+// r -= alpha A p and the per-block |r|^2 (the solution step is lagged)
+template <typename SR, typename SA>
+__global__ void __launch_bounds__(256) k_cg_r_t(FieldRef r, FieldRef ap, const double* alpha, const int* active,
+                                                int64_t per_rhs, double* partial) {
+  ELEM_SETUP
+  double n = 0;
+  if (active[rhs]) {
+    using R = float;
+    const R a = R(alpha[rhs]);
+    cplx<typename SpinTraits<SR>::R> vr[12];
+    cplx<typename SpinTraits<SA>::R> va[12];
+    spin_load<SR, typename SpinTraits<SR>::R>(r, e, vr);
+    spin_load<SA, typename SpinTraits<SA>::R>(ap, e, va);
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      vr[c].re -= a * R(va[c].re); vr[c].im -= a * R(va[c].im);
+      n += double(vr[c].re) * vr[c].re + double(vr[c].im) * vr[c].im;
+    }
     spin_store<SR, typename SpinTraits<SR>::R>(r, e, vr);
   }
   block_partial_d(n, partial, slot);
@@ -388,6 +449,33 @@ hipError_t blas_xpay_mixed(int sp, const FieldRef& p, int sr, const FieldRef& r,
   if (sp == STORE_H && sr == STORE_H) { hipLaunchKernelGGL((k_xpay_t<SpinH, SpinH>), g, b, 0, s, p, r, beta, active, per_rhs); return hipGetLastError(); }
   if (sp == STORE_F && sr == STORE_F) { hipLaunchKernelGGL((k_xpay_t<SpinF, SpinF>), g, b, 0, s, p, r, beta, active, per_rhs); return hipGetLastError(); }
   return hipErrorInvalidValue;
+}
+
+hipError_t blas_xpay_x_mixed(int sp, const FieldRef& p, int sr, const FieldRef& r, int sx, const FieldRef& x,
+                             const double* alpha, const double* beta, const int* active, int64_t per_rhs, int nrhs,
+                             int bs, hipStream_t s) {
+  dim3 g = egrid(per_rhs, bs, nrhs), b(bs);
+#define CPSHIP_XPX(P, TP, RR, TR, X, TX)                                                                     \
+  if (sp == P && sr == RR && sx == X) {                                                                      \
+    hipLaunchKernelGGL((k_xpay_x_t<TP, TR, TX>), g, b, 0, s, p, r, x, alpha, beta, active, per_rhs);         \
+    return hipGetLastError();                                                                                \
+  }
+  CPSHIP_XPX(STORE_H, SpinH, STORE_F, SpinF, STORE_F, SpinF)
+  CPSHIP_XPX(STORE_H, SpinH, STORE_F, SpinF, STORE_H, SpinH)
+  CPSHIP_XPX(STORE_H, SpinH, STORE_H, SpinH, STORE_H, SpinH)
+  CPSHIP_XPX(STORE_F, SpinF, STORE_F, SpinF, STORE_F, SpinF)
+#undef CPSHIP_XPX
+  return hipErrorInvalidValue;
+}
+
+hipError_t blas_cg_r_mixed(int sr, const FieldRef& r, int sa, const FieldRef& ap, const CGScalars& cg,
+                           int64_t per_rhs, int nrhs, int bs, double* partial, hipStream_t s) {
+  dim3 g = egrid(per_rhs, bs, nrhs), b(bs);
+  if (sr == STORE_F && sa == STORE_H) hipLaunchKernelGGL((k_cg_r_t<SpinF, SpinH>), g, b, 0, s, r, ap, cg.alpha, cg.active, per_rhs, partial);
+  else if (sr == STORE_H && sa == STORE_H) hipLaunchKernelGGL((k_cg_r_t<SpinH, SpinH>), g, b, 0, s, r, ap, cg.alpha, cg.active, per_rhs, partial);
+  else if (sr == STORE_F && sa == STORE_F) hipLaunchKernelGGL((k_cg_r_t<SpinF, SpinF>), g, b, 0, s, r, ap, cg.alpha, cg.active, per_rhs, partial);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 hipError_t blas_zero(int st, const FieldRef& x, int64_t per_rhs, int nrhs, int bs, hipStream_t s) {

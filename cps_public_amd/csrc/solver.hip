@@ -856,6 +856,9 @@ static void mixed_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int*
 __global__ void k_mixed_after_update(CGScalars cg, const double* rr_true, const double* rp, double* coef, int nrhs) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nrhs) return;
+  // rp aliases alpha: read it, then leave no solution step pending
+  const double rpk = rp[k];
+  cg.alpha[k] = 0.0;
   coef[k] = 0.0;
   const int st = cg.status[k];
   if (st == CG_STATUS_D_ZERO || st == CG_STATUS_MAXITER) return;
@@ -870,7 +873,7 @@ __global__ void k_mixed_after_update(CGScalars cg, const double* rr_true, const 
     return;
   }
   cg.beta[k] = v / cg.rr_prev[k];
-  coef[k] = v > 0.0 ? -rp[k] / v : 0.0;
+  coef[k] = v > 0.0 ? -rpk / v : 0.0;
   cg.rr[k] = v;
   cg.active[k] = 1;
   cg.status[k] = CG_STATUS_RUNNING;
@@ -934,10 +937,20 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   CPSHIP_CHECK(convert_spinor(STORE_D, r.ref(), sps, ps.ref(), n, L.stream));
   CPSHIP_CHECK(blas_zero(xvec, xs.ref(), o.per_rhs, nrhs, bsb, L.stream));
   {
-    // the first iteration's direction update p = r + 0 p
+    // the first iteration's direction update p = r + 0 p, and no pending
+    // solution step
     std::vector<double> zeros(nrhs, 0.0);
     CPSHIP_CHECK(hipMemcpyAsync(cg.s.beta, zeros.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, L.stream));
+    CPSHIP_CHECK(hipMemcpyAsync(cg.s.alpha, zeros.data(), nrhs * sizeof(double), hipMemcpyHostToDevice, L.stream));
   }
+  // the solution step x += alpha p is applied one iteration late, by the
+  // kernel that reads p next (the direction update), which saves one read of
+  // p per iteration; a reliable update applies the pending step first
+  // (CPS_HIP_LAGGED_X=0: in the same iteration, with r)
+  const bool lagged = [] {
+    const char* e = getenv("CPS_HIP_LAGGED_X");
+    return !(e && *e == '0');
+  }();
   DevBuf coef_buf;
   if ((rc = coef_buf.ensure(nrhs * sizeof(double)))) return rc;
   double* coef = (double*)coef_buf.p;
@@ -972,12 +985,19 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     // first, then Ap, alpha, x and r, |r|^2 and beta -- so that a reliable
     // update at a chunk boundary falls between r's update and p's
     rc = iter_graph.run(L, todo, [&]() -> int {
-      CPSHIP_CHECK(blas_xpay_mixed(sps, psw, svec, rsw, sw.beta, sw.active, o.per_rhs, nw, bsb, L.stream));
+      if (lagged)
+        CPSHIP_CHECK(blas_xpay_x_mixed(sps, psw, svec, rsw, xvec, xsw, sw.alpha, sw.beta, sw.active, o.per_rhs, nw,
+                                       bsb, L.stream));
+      else
+        CPSHIP_CHECK(blas_xpay_mixed(sps, psw, svec, rsw, sw.beta, sw.active, o.per_rhs, nw, bsb, L.stream));
       int e = apply_mdagm(L, op, hprec, apsw, psw, t1s.ref(), t2s.ref(), part, nw);
       if (e) return e;
       CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
-      CPSHIP_CHECK(blas_cg_xr_mixed(xvec, xsw, sps, psw, svec, rsw, sps, apsw, sw, o.per_rhs, nw, bsb, part,
-                                    L.stream));
+      if (lagged)
+        CPSHIP_CHECK(blas_cg_r_mixed(svec, rsw, sps, apsw, sw, o.per_rhs, nw, bsb, part, L.stream));
+      else
+        CPSHIP_CHECK(blas_cg_xr_mixed(xvec, xsw, sps, psw, svec, rsw, sps, apsw, sw, o.per_rhs, nw, bsb, part,
+                                      L.stream));
       CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, nullptr, RED_CG_BETA, sw, L.stream));
       return 0;
     });
@@ -1003,6 +1023,8 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     if (since >= every || paused || done >= max_itr) {
       // ---- FP64-critical steps: accumulate the solution, recompute r ----
       // (on the window: the rhs outside it have converged and keep x)
+      if (lagged)   // the pending step x += alpha p of the last iteration
+        CPSHIP_CHECK(blas_axpy(sps, psw, xvec, xsw, sw.alpha, 0.0, o.per_rhs, nw, bsb, L.stream));
       CPSHIP_CHECK(blas_axpy(xvec, xsw, STORE_D, xw, nullptr, 1.0, o.per_rhs, nw, bsb, L.stream));
       CPSHIP_CHECK(blas_zero(xvec, xsw, o.per_rhs, nw, bsb, L.stream));
       rc = apply_mdagm(L, op, PREC_D_D18, apw, xw, t1.ref(), t2.ref(), nullptr, nw);
