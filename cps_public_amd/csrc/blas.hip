@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(256) k_axpy(FieldRef x, FieldRef y, const doub
   spin_load<SX, RX>(x, e, vx);
   spin_load<SY, RY>(y, e, vy);
 #pragma unroll
-  for (int k = 0; k < 12; k++) vy[k] = cmk<RY>(c * RY(vx[k].re) + vy[k].re, c * RY(vx[k].im) + vy[k].im);
+  for (int k = 0; k < 12; k++) vy[k] = cmk<RY>(fma_r(c, RY(vx[k].re), vy[k].re), fma_r(c, RY(vx[k].im), vy[k].im));
   spin_store<SY, RY>(y, e, vy);
 }
 
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) k_cg_xr_t(FieldRef x, FieldRef p, FieldRe
     spin_load<SA, typename SpinTraits<SA>::R>(ap, e, va);
 #pragma unroll
     for (int c = 0; c < 12; c++) {
-      vx[c].re += a * R(vp[c].re); vx[c].im += a * R(vp[c].im);
+      vx[c].re = fma_r(a, R(vp[c].re), vx[c].re); vx[c].im = fma_r(a, R(vp[c].im), vx[c].im);
       vr[c].re -= a * R(va[c].re); vr[c].im -= a * R(va[c].im);
       n += double(vr[c].re) * vr[c].re + double(vr[c].im) * vr[c].im;
     }
@@ -213,8 +213,8 @@ __global__ void __launch_bounds__(256) k_xpay_x_t(FieldRef p, FieldRef r, FieldR
     spin_load<SX, typename SpinTraits<SX>::R>(x, e, vx);
 #pragma unroll
     for (int c = 0; c < 12; c++) {
-      vx[c].re += ar * R(vp[c].re);
-      vx[c].im += ar * R(vp[c].im);
+      vx[c].re = fma_r(ar, R(vp[c].re), vx[c].re);
+      vx[c].im = fma_r(ar, R(vp[c].im), vx[c].im);
     }
     spin_store<SX, typename SpinTraits<SX>::R>(x, e, vx);
   }
@@ -224,8 +224,63 @@ __global__ void __launch_bounds__(256) k_xpay_x_t(FieldRef p, FieldRef r, FieldR
     spin_load<SR, typename SpinTraits<SR>::R>(r, e, vr);
     cplx<R> o[12];
 #pragma unroll
-    for (int c = 0; c < 12; c++) o[c] = cmk<R>(b * R(vp[c].re) + R(vr[c].re), b * R(vp[c].im) + R(vr[c].im));
+    for (int c = 0; c < 12; c++) o[c] = cmk<R>(fma_r(b, R(vp[c].re), R(vr[c].re)), fma_r(b, R(vp[c].im), R(vr[c].im)));
     spin_store<SP, R>(p, e, o);
+  }
+}
+
+// The same for the Moebius layout with Ls = 16, writing also
+// t5 = b5 p + c5 P p (P of dag 0: upper spins from s-1, lower from s+1, -mass
+// at the walls), the first step of the Moebius M, from the stored (FP16) p:
+// the 16 slots of a site are one 16-lane row, so the slots s-1, s+1 come
+// from the neighbouring lanes.  The arithmetic is that of k_s5.
+__device__ __forceinline__ float ror_f(float v, int n) {
+  return n == 1 ? __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x121, 0xF, 0xF, true))
+                : __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x12F, 0xF, 0xF, true));
+}
+
+__global__ void __launch_bounds__(256) k_xpay_x_s5(FieldRef p, FieldRef r, FieldRef x, FieldRef t5, const double* alpha,
+                                                   const double* beta, const int* active, int64_t per_rhs, float b5,
+                                                   float c5, float mass) {
+  const int rhs = blockIdx.y;
+  const double a = alpha[rhs];
+  const bool act = active[rhs] != 0;
+  if (!act && a == 0.0) return;
+  const int64_t e = (int64_t)rhs * per_rhs + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  using R = float;
+  cplx<R> vp[12];
+  spin_load<SpinH, R>(p, e, vp);
+  if (a != 0.0) {
+    const R ar = R(a);
+    cplx<R> vx[12];
+    spin_load<SpinF, R>(x, e, vx);
+#pragma unroll
+    for (int c = 0; c < 12; c++) {
+      vx[c].re = fma_r(ar, R(vp[c].re), vx[c].re);
+      vx[c].im = fma_r(ar, R(vp[c].im), vx[c].im);
+    }
+    spin_store<SpinF, R>(x, e, vx);
+  }
+  if (act) {
+    const R b = R(beta[rhs]);
+    cplx<R> vr[12];
+    spin_load<SpinF, R>(r, e, vr);
+    cplx<R> o[12], pr[12], t[12];
+#pragma unroll
+    for (int c = 0; c < 12; c++) o[c] = cmk<R>(fma_r(b, R(vp[c].re), R(vr[c].re)), fma_r(b, R(vp[c].im), R(vr[c].im)));
+    spin_store<SpinH, R>(p, e, o);
+    half_roundtrip(o, pr);
+    const int s = (int)(e & 15);
+    const R cm = (s == 0) ? -mass : R(1), cp = (s == 15) ? -mass : R(1);
+    const R one = R(1);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      t[k] = cmk<R>(one * fma_r(b5, pr[k].re, c5 * (cm * ror_f(pr[k].re, 1))),
+                    one * fma_r(b5, pr[k].im, c5 * (cm * ror_f(pr[k].im, 1))));
+      t[6 + k] = cmk<R>(one * fma_r(b5, pr[6 + k].re, c5 * (cp * ror_f(pr[6 + k].re, 15))),
+                        one * fma_r(b5, pr[6 + k].im, c5 * (cp * ror_f(pr[6 + k].im, 15))));
+    }
+    spin_store<SpinH, R>(t5, e, t);
   }
 }
 
@@ -266,7 +321,7 @@ __global__ void __launch_bounds__(256) k_xpay_t(FieldRef p, FieldRef r, const do
   spin_load<SR, typename SpinTraits<SR>::R>(r, e, vr);
   cplx<R> o[12];
 #pragma unroll
-  for (int c = 0; c < 12; c++) o[c] = cmk<R>(b * R(vp[c].re) + R(vr[c].re), b * R(vp[c].im) + R(vr[c].im));
+  for (int c = 0; c < 12; c++) o[c] = cmk<R>(fma_r(b, R(vp[c].re), R(vr[c].re)), fma_r(b, R(vp[c].im), R(vr[c].im)));
   spin_store<SP, R>(p, e, o);
 }
 
@@ -466,6 +521,16 @@ hipError_t blas_xpay_x_mixed(int sp, const FieldRef& p, int sr, const FieldRef& 
   CPSHIP_XPX(STORE_F, SpinF, STORE_F, SpinF, STORE_F, SpinF)
 #undef CPSHIP_XPX
   return hipErrorInvalidValue;
+}
+
+hipError_t blas_xpay_x_s5(const FieldRef& p, const FieldRef& r, const FieldRef& x, const FieldRef& t5,
+                          const double* alpha, const double* beta, const int* active, int64_t per_rhs, int nrhs, int bs,
+                          double b5, double c5, double mass, hipStream_t s) {
+  if (per_rhs % 16 || bs % 64) return hipErrorInvalidValue;   // rows of 16 slots within whole wavefronts
+  dim3 g = egrid(per_rhs, bs, nrhs), b(bs);
+  hipLaunchKernelGGL(k_xpay_x_s5, g, b, 0, s, p, r, x, t5, alpha, beta, active, per_rhs, float(b5), float(c5),
+                     float(mass));
+  return hipGetLastError();
 }
 
 hipError_t blas_cg_r_mixed(int sr, const FieldRef& r, int sa, const FieldRef& ap, const CGScalars& cg,

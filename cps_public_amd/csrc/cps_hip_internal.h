@@ -222,6 +222,26 @@ __device__ __forceinline__ void spin_store(const FieldRef& f, int64_t e, const c
   }
 }
 
+template <typename R>
+__device__ __forceinline__ R fma_r(R a, R b, R c) {
+  if constexpr (std::is_same<R, float>::value) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
+
+// The values spin_load<SpinH> returns after spin_store<SpinH> of v (the FP16
+// quantisation with the element's norm, undone)
+__device__ __forceinline__ void half_roundtrip(const cplx<float> v[12], cplx<float> r[12]) {
+  float m = 0.f;
+#pragma unroll
+  for (int c = 0; c < 12; c++) m = fmaxf(m, fmaxf(fabsf(v[c].re), fabsf(v[c].im)));
+  const float inv = m >= 1.17549435e-38f ? 1.0f / m : 0.f;
+#pragma unroll
+  for (int c = 0; c < 12; c++) {
+    const float2 h = h2f(f2h(v[c].re * inv, v[c].im * inv));
+    r[c] = cmk<float>(h.x * m, h.y * m);
+  }
+}
+
 // ---- gauge storage ------------------------------------------------------------
 struct LinkD18 {};  // 9 planes of double2
 struct LinkF12 {};  // 3 planes of float4 (rows 0,1)

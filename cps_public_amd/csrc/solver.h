@@ -195,16 +195,17 @@ int tuned_launch(Lattice& L, HopParams& P);
 void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp_store, int* hprec);
 int apply_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, const FieldRef& t,
             int dag, double* partial, int nrhs);
+// t1_ready (Moebius): t1 already holds b in + c P in, the first step of M
 int apply_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
-                const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs);
+                const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs, bool t1_ready = false);
 int apply_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
                 const FieldRef& t1, const FieldRef& t2, int dag, int nrhs);
 
 // Moebius (mobius.hip) and clover (clover.hip)
 int mobius_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
-             const FieldRef& t1, const FieldRef& t2, int dag, double* partial, int nrhs);
+             const FieldRef& t1, const FieldRef& t2, int dag, double* partial, int nrhs, bool t1_ready = false);
 int mobius_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
-                 const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs);
+                 const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs, bool t1_ready = false);
 int clover_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
                  const FieldRef& t1, int dag, double* partial, int nrhs);
 int clover_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,

@@ -491,8 +491,8 @@ int apply_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, co
 
 // MatPcDagMatPc: out = M^dag M in; partial (optional) = |M in|^2 per block
 int apply_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
-                const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs) {
-  if (op->kind == CPS_HIP_OP_MOBIUS) return mobius_mdagm(L, op, hprec, out, in, t1, t2, partial, nrhs);
+                const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs, bool t1_ready) {
+  if (op->kind == CPS_HIP_OP_MOBIUS) return mobius_mdagm(L, op, hprec, out, in, t1, t2, partial, nrhs, t1_ready);
   if (op->kind == CPS_HIP_OP_CLOVER) return clover_mdagm(L, op, hprec, out, in, t1, t2, partial, nrhs);
   int rc = apply_m(L, op, hprec, t2, in, t1, 0, partial, nrhs);
   if (rc) return rc;
@@ -951,6 +951,14 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     const char* e = getenv("CPS_HIP_LAGGED_X");
     return !(e && *e == '0');
   }();
+  // Moebius, FP16 directions, Ls 16: the direction update also writes the
+  // first step of M, b p + c P p, into the operator's first scratch field
+  // (CPS_HIP_S5_XPAY=0: the operator computes it)
+  const bool s5_in_xpay = [&] {
+    const char* e = getenv("CPS_HIP_S5_XPAY");
+    return lagged && !(e && *e == '0') && op->kind == CPS_HIP_OP_MOBIUS && sps == STORE_H && svec == STORE_F &&
+           xvec == STORE_F && L.ls == 16;
+  }();
   DevBuf coef_buf;
   if ((rc = coef_buf.ensure(nrhs * sizeof(double)))) return rc;
   double* coef = (double*)coef_buf.p;
@@ -985,12 +993,15 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     // first, then Ap, alpha, x and r, |r|^2 and beta -- so that a reliable
     // update at a chunk boundary falls between r's update and p's
     rc = iter_graph.run(L, todo, [&]() -> int {
-      if (lagged)
+      if (s5_in_xpay)
+        CPSHIP_CHECK(blas_xpay_x_s5(psw, rsw, xsw, t1s.ref(), sw.alpha, sw.beta, sw.active, o.per_rhs, nw, bsb,
+                                    op->mobius_b, op->mobius_c, op->mass, L.stream));
+      else if (lagged)
         CPSHIP_CHECK(blas_xpay_x_mixed(sps, psw, svec, rsw, xvec, xsw, sw.alpha, sw.beta, sw.active, o.per_rhs, nw,
                                        bsb, L.stream));
       else
         CPSHIP_CHECK(blas_xpay_mixed(sps, psw, svec, rsw, sw.beta, sw.active, o.per_rhs, nw, bsb, L.stream));
-      int e = apply_mdagm(L, op, hprec, apsw, psw, t1s.ref(), t2s.ref(), part, nw);
+      int e = apply_mdagm(L, op, hprec, apsw, psw, t1s.ref(), t2s.ref(), part, nw, s5_in_xpay);
       if (e) return e;
       CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
       if (lagged)

@@ -245,6 +245,29 @@ def test_mixed_lagged_solution_step_bit_identical(monkeypatch, kind):
     assert max(r.true_res for r in out["1"][1]) <= 1e-10
 
 
+def test_mixed_s5_in_direction_update_bit_identical(monkeypatch):
+    """Moebius, Ls 16, FP16 directions: the direction update writing the
+    first step of M (b p + c P p, slots s-1, s+1 from the neighbouring lanes)
+    gives the same solve, bit for bit, as the operator's own k_s5 pass."""
+    L, ls = [4, 4, 4, 8], 16
+    op = C.make_op(C.OP_MOBIUS, mass=0.02, m5=1.8, b=1.5, c=0.5)
+    u = wilson_gauge(L, 0.2, 290)
+    vcb = int(np.prod(L)) // 2
+    src = np.concatenate([F.gaussian_fermion(vcb * ls, 291 + k) for k in range(2)])
+    arg = C.make_cg_arg(1e-10, 5000, precision=C.PREC_MIXED_HALF)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("CPS_HIP_S5_XPAY", fused)
+        lat = C.Lattice(L, ls, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        out[fused] = lat.invcg(op, src, arg, nrhs=2)
+        lat.close()
+    print("s5 in xpay", [r.iterations for r in out["1"][1]], [r.iterations for r in out["0"][1]], flush=True)
+    assert [r.iterations for r in out["1"][1]] == [r.iterations for r in out["0"][1]]
+    assert np.array_equal(out["1"][0], out["0"][0])
+    assert max(r.true_res for r in out["1"][1]) <= 1e-10
+
+
 def test_invcg_device_matches_host(dwf4):
     """cps_hip_invcg_device (vectors resident in HBM) gives the host entry
     point's result bit for bit, for FP64 and mixed precision, two rhs."""
