@@ -71,6 +71,13 @@ struct HopParams {
   // hop come from the neighbouring lanes of the site's 16-lane row (DPP
   // rotations) instead of loads and lane permutes
   int s5_row;
+  // rupd (with s5_row): out is not stored; instead the CG residual R (FP32)
+  // of every rhs with ractive[rhs] becomes R - ralpha[rhs] * out, and the
+  // partials are |R|^2 (0 for the other rhs)
+  int rupd;
+  FieldRef R;
+  const double* ralpha;
+  const int* ractive;
   const uint4* nbr;         // neighbour table (k_build_nbr)
   const uint32_t* nbr_neg;
   // HOP_SPATIAL: site_cb of logical site b*perm_S + q (the same table for
@@ -100,9 +107,10 @@ struct HopParams {
   const double* kscale;
 };
 
-// CLOV_S5XR: CLOV_S5X with the row exchange of s5_row (a kernel of its own,
-// chosen by the launcher; not a value of HopParams::clov)
-enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2, CLOV_S5X = 3, CLOV_S5XR = 4 };
+// CLOV_S5XR: CLOV_S5X with the row exchange of s5_row; CLOV_S5XRR: the same
+// with the CG residual update (rupd) instead of storing out (kernels of their
+// own, chosen by the launcher; not values of HopParams::clov)
+enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2, CLOV_S5X = 3, CLOV_S5XR = 4, CLOV_S5XRR = 5 };
 
 hipError_t launch_hop(const HopParams& P, hipStream_t st);
 hipError_t build_nbr(const Geom& g, uint4* nbr, uint32_t* nbr_neg, hipStream_t st);

@@ -835,14 +835,15 @@ __device__ __forceinline__ void m5_row(const HopParams& P, int s, cplx<float> (&
 }
 
 // One output element: the hop, the fifth-dimension term (DWF), the epilogue
-// and the store; returns the element's |out|^2 when partials are requested.
+// and the store; returns the element's |out|^2 when partials are requested
+// (|r|^2 of the updated residual for CLOV_S5XRR), summed in FP64.
 template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV, int LDSL>
-__device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, const Nbr& N, uint32_t* lds) {
+__device__ __forceinline__ double hop_element(const HopParams& P, const HopElem& E, const Nbr& N, uint32_t* lds) {
   const int rhs = E.rhs, site = E.site, j = E.j;
   const int64_t per_rhs = (int64_t)P.nsites_out * P.nb;
   const int64_t rem = (int64_t)site * P.nb + j;
   const bool valid = rhs < P.nrhs;
-  R nrm = R(0);
+  double nrm = 0.0;
   const Geom& g = P.geom;
   int out_par, site_cb;
   if (KIND == KIND_DWF) {       // fields hold both 4-D parities
@@ -926,7 +927,7 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
 
     const int64_t eo = (int64_t)rhs * per_rhs + rem;
     cplx<R> res[12];
-    if constexpr (CLOV == CLOV_S5X || CLOV == CLOV_S5XR) {
+    if constexpr (CLOV == CLOV_S5X || CLOV == CLOV_S5XR || CLOV == CLOV_S5XRR) {
       // j = s of the Moebius layout; the Ls slots of this site are adjacent
       const int ls = P.nb;
       const int64_t sbase = eo - j;
@@ -940,7 +941,7 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
       constexpr int c_from_p = DAG ? 0 : 6;
       cplx<R> xv[12], hm[6], hp[6];
       spin_load<SOUT, R>(P.X, eo, xv);
-      constexpr bool ROW = CLOV == CLOV_S5XR && std::is_same<R, float>::value;
+      constexpr bool ROW = (CLOV == CLOV_S5XR || CLOV == CLOV_S5XRR) && std::is_same<R, float>::value;
       if constexpr (ROW) {
 #pragma unroll
         for (int k = 0; k < 6; k++) {
@@ -1026,10 +1027,26 @@ __device__ __forceinline__ R hop_element(const HopParams& P, const HopElem& E, c
         if (P.m5post) m5_row<DAG>(P, j, res);
       }
     }
-    spin_store<SOUT, R, NT>(P.out, eo, res);
-    if (P.partial) {
+    if constexpr (CLOV == CLOV_S5XRR) {
+      // the CG step r -= alpha (M^dag M p) and |r|^2, with M^dag M p in FP32
+      if (P.ractive[rhs]) {
+        const R a = R(P.ralpha[rhs]);
+        cplx<R> rv[12];
+        spin_load<SpinF, R>(P.R, eo, rv);
 #pragma unroll
-      for (int k = 0; k < 12; k++) nrm += res[k].re * res[k].re + res[k].im * res[k].im;
+        for (int k = 0; k < 12; k++) {
+          rv[k].re -= a * res[k].re;
+          rv[k].im -= a * res[k].im;
+          nrm += double(rv[k].re) * rv[k].re + double(rv[k].im) * rv[k].im;
+        }
+        spin_store<SpinF, R>(P.R, eo, rv);
+      }
+    } else {
+      spin_store<SOUT, R, NT>(P.out, eo, res);
+      if (P.partial) {
+#pragma unroll
+        for (int k = 0; k < 12; k++) nrm += double(res[k].re) * res[k].re + double(res[k].im) * res[k].im;
+      }
     }
   }
   return nrm;
@@ -1049,9 +1066,9 @@ __global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) 
   const HopElem E = SW == 1 ? hop_decode_sitewave(P, hb, nsub)
                    : SW == 3 ? hop_decode_rhsgroup(P, hb) : hop_decode<KIND>(P, hb, nsub);
   const Nbr N = hop_nbr<KIND>(P, E);
-  const R nrm = hop_element<SIN, SOUT, LK, R, DAG, KIND, NT, CLOV, SW == 2>(
+  const double nrm = hop_element<SIN, SOUT, LK, R, DAG, KIND, NT, CLOV, SW == 2>(
       P, E, N, reinterpret_cast<uint32_t*>(hop_lds_area));
-  if (P.partial) block_partial<R>(nrm, P.partial, E.pslot);
+  if (P.partial) block_partial<double>(nrm, P.partial, E.pslot);
 }
 
 }  // namespace cpship

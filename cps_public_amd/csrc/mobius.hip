@@ -521,6 +521,13 @@ struct M5Post {
   double K, mass, b, c;
 };
 
+// the CG residual update fused into the last hop of M^dag (HopParams::rupd)
+struct RUpdate {
+  FieldRef r;
+  const double* alpha;
+  const int* active;
+};
+
 // CPS_HIP_M5_FUSED=0 keeps M5^-1 as its own pass
 static bool m5_fused_enabled() {
   const char* e = getenv("CPS_HIP_M5_FUSED");
@@ -536,7 +543,8 @@ static bool s5_fused_enabled() {
 // s5x: out = X + x5coef (P X) + scale (hb5 hop + hc5 P hop) (CLOV_S5X)
 static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag, double scale,
                 int nrhs, bool s5x = false, const FieldRef* X = nullptr, double x5coef = 0.0, double mass = 0.0,
-                double* partial = nullptr, double hb5 = 1.0, double hc5 = 0.0, const M5Post* m5 = nullptr) {
+                double* partial = nullptr, double hb5 = 1.0, double hc5 = 0.0, const M5Post* m5 = nullptr,
+                const RUpdate* ru = nullptr) {
   cps_hip_op w;
   std::memset(&w, 0, sizeof(w));
   w.kind = CPS_HIP_OP_MOBIUS;   // single-parity layout with nb = Ls
@@ -568,6 +576,13 @@ static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, 
     // CPS_HIP_S5X_ROW=0 loads the slots s-1, s+1 of X instead
     const char* e = getenv("CPS_HIP_S5X_ROW");
     P.s5_row = (store_of(hprec) == STORE_H && L.ls == 16 && !(e && *e == '0')) ? 1 : 0;
+  }
+  if (ru) {
+    P.s5_row = 1;   // the residual step exists in the row variant only
+    P.rupd = 1;
+    P.R = ru->r;
+    P.ralpha = ru->alpha;
+    P.ractive = ru->active;
   }
   if (m5) {
     P.m5post = 1;
@@ -643,6 +658,26 @@ int mobius_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, c
   P.B = in; P.a2 = 1; P.b2 = 1; P.c2 = ratio;
   P.partial = partial;
   return launch_s5(L, hprec, P);
+}
+
+// r -= alpha M^dag t3 for the rhs with active set, and the partials |r|^2:
+// the second half of a CG iteration's M^dag M with the residual step in the
+// epilogue of M^dag's last hop (half precision, Ls 16: both fifth-dimension
+// stages run in hop epilogues, so M^dag is two hops)
+int mobius_mdag_rupdate(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& r, const FieldRef& t3,
+                        const FieldRef& t1, const FieldRef& t2, const double* alpha, const int* active,
+                        double* partial, int nrhs) {
+  (void)t1;
+  if (store_of(hprec) != STORE_H || L.ls != 16 || !m5_fused_enabled() || !s5_fused_enabled()) return CPS_HIP_ERR_ARG;
+  double kb, kc;
+  mobius_kappas(op, &kb, &kc);
+  const double ratio = kb / kc;
+  const M5Post m5{ratio, op->mass, op->mobius_b, op->mobius_c};
+  int rc = hop4(L, hprec, t2, t3, 0, 1, -kb, nrhs, false, nullptr, 0.0, 0.0, nullptr, 1.0, 0.0, &m5);
+  if (rc) return rc;
+  const RUpdate ru{r, alpha, active};
+  return hop4(L, hprec, r, t2, 1, 1, kb, nrhs, true, &t3, ratio, op->mass, partial, op->mobius_b, op->mobius_c,
+              nullptr, &ru);
 }
 
 // mobius_mdagm (mobius_mdagm.C:33-60): t3 = M in, |t3|^2, out = M^dag t3.

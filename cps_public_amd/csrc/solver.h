@@ -206,6 +206,9 @@ int mobius_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, c
              const FieldRef& t1, const FieldRef& t2, int dag, double* partial, int nrhs, bool t1_ready = false);
 int mobius_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
                  const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs, bool t1_ready = false);
+int mobius_mdag_rupdate(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& r, const FieldRef& t3,
+                        const FieldRef& t1, const FieldRef& t2, const double* alpha, const int* active,
+                        double* partial, int nrhs);
 int clover_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
                  const FieldRef& t1, int dag, double* partial, int nrhs);
 int clover_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,

@@ -363,7 +363,7 @@ int tuned_launch(Lattice& L, HopParams& P) {
   if (const char* m = getenv("CPS_HIP_HOP_MODE"))
     mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3 : !strcmp(m, "rhsgroup") ? 4 : 0;
   HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr,
-                 P.clov + 16 * P.m5post, {L.g.L[0], L.g.L[1], L.g.L[2], L.g.L[3]}, mode};
+                 P.clov + 16 * P.m5post + 32 * P.rupd, {L.g.L[0], L.g.L[1], L.g.L[2], L.g.L[3]}, mode};
   // the winners are kept per process (every handle, engine and thread of the
   // same geometry reuses them); a launch shape is timed once
   static std::mutex tune_mu;
@@ -403,7 +403,9 @@ int tuned_launch(Lattice& L, HopParams& P) {
     // one site per lane the spatial sweep is the slower order: 6.2 against
     // 4.6 ms for the 32^3x64 clover hop with 48 rhs)
     HopTuneVal best{def_block, (P.nb >= 8 && spatial_ok(L, P, def_block)) ? HOP_SPATIAL : 0, 0.0};
-    if (tuning_enabled()) {
+    // a launch that updates a field in place (rupd) is never timed on the
+    // live data: it keeps the fixed order of launches with partials
+    if (tuning_enabled() && !P.rupd) {
       std::vector<int> blocks;
       if (P.partial) blocks.push_back(def_block);
       else
@@ -959,6 +961,15 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     return lagged && !(e && *e == '0') && op->kind == CPS_HIP_OP_MOBIUS && sps == STORE_H && svec == STORE_F &&
            xvec == STORE_F && L.ls == 16;
   }();
+  // with it, the residual step r -= alpha A p runs in the epilogue of the
+  // last hop of M^dag (alpha is known after the first half of M^dag M), so A p
+  // is never stored (CPS_HIP_R_IN_HOP=0: a separate pass)
+  const bool r_in_hop = [&] {
+    const char* e = getenv("CPS_HIP_R_IN_HOP");
+    return s5_in_xpay && !(e && *e == '0');
+  }();
+  Field& t3s = L.field_min(S_T3S, sps, n);
+  if (L.last_rc) return L.last_rc;
   DevBuf coef_buf;
   if ((rc = coef_buf.ensure(nrhs * sizeof(double)))) return rc;
   double* coef = (double*)coef_buf.p;
@@ -1001,6 +1012,16 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
                                        bsb, L.stream));
       else
         CPSHIP_CHECK(blas_xpay_mixed(sps, psw, svec, rsw, sw.beta, sw.active, o.per_rhs, nw, bsb, L.stream));
+      if (r_in_hop) {
+        // M p with |M p|^2, alpha, then r -= alpha M^dag (M p) with |r|^2
+        int e = mobius_m(L, op, hprec, t3s.ref(), psw, t1s.ref(), t2s.ref(), 0, part, nw, true);
+        if (e) return e;
+        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
+        e = mobius_mdag_rupdate(L, op, hprec, rsw, t3s.ref(), t1s.ref(), t2s.ref(), sw.alpha, sw.active, part, nw);
+        if (e) return e;
+        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_BETA, sw, L.stream));
+        return 0;
+      }
       int e = apply_mdagm(L, op, hprec, apsw, psw, t1s.ref(), t2s.ref(), part, nw, s5_in_xpay);
       if (e) return e;
       CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));

@@ -232,6 +232,7 @@ def test_mixed_lagged_solution_step_bit_identical(monkeypatch, kind):
     vcb = int(np.prod(L)) // 2
     src = np.concatenate([F.gaussian_fermion(vcb * ls, 281 + k) * (1.0 + 9.0 * k) for k in range(3)])
     arg = C.make_cg_arg(1e-10, 5000, precision=C.PREC_MIXED_HALF, reliable_every=30)
+    monkeypatch.setenv("CPS_HIP_R_IN_HOP", "0")   # (it rounds A p differently)
     out = {}
     for lag in ("1", "0"):
         monkeypatch.setenv("CPS_HIP_LAGGED_X", lag)
@@ -255,6 +256,7 @@ def test_mixed_s5_in_direction_update_bit_identical(monkeypatch):
     vcb = int(np.prod(L)) // 2
     src = np.concatenate([F.gaussian_fermion(vcb * ls, 291 + k) for k in range(2)])
     arg = C.make_cg_arg(1e-10, 5000, precision=C.PREC_MIXED_HALF)
+    monkeypatch.setenv("CPS_HIP_R_IN_HOP", "0")   # (it rounds A p differently)
     out = {}
     for fused in ("1", "0"):
         monkeypatch.setenv("CPS_HIP_S5_XPAY", fused)
@@ -266,6 +268,40 @@ def test_mixed_s5_in_direction_update_bit_identical(monkeypatch):
     assert [r.iterations for r in out["1"][1]] == [r.iterations for r in out["0"][1]]
     assert np.array_equal(out["1"][0], out["0"][0])
     assert max(r.true_res for r in out["1"][1]) <= 1e-10
+
+
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_mixed_residual_step_in_hop(oracle, monkeypatch, nrhs):
+    """Moebius, Ls 16, half precision: r -= alpha M^dag M p and |r|^2 in the
+    epilogue of M^dag's last hop (alpha from the first half), A p never
+    stored.  It rounds A p differently (FP32 instead of FP16), so the check
+    is the FP64 stopping rule, the oracle's solution, and iteration counts
+    within 5% of the separate residual pass."""
+    L, ls = [4, 4, 4, 8], 16
+    op = C.make_op(C.OP_MOBIUS, mass=0.02, m5=1.8, b=1.5, c=0.5)
+    u = wilson_gauge(L, 0.2, 300)
+    vcb = int(np.prod(L)) // 2
+    n = 24 * vcb * ls
+    src = np.concatenate([F.gaussian_fermion(vcb * ls, 301 + k) * (1.0 + 4.0 * k) for k in range(nrhs)])
+    arg = C.make_cg_arg(1e-10, 5000, precision=C.PREC_MIXED_HALF, reliable_every=40)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("CPS_HIP_R_IN_HOP", fused)
+        lat = C.Lattice(L, ls, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        out[fused] = lat.invcg(op, src, arg, nrhs=nrhs)
+        lat.close()
+    it1 = [r.iterations for r in out["1"][1]]
+    it0 = [r.iterations for r in out["0"][1]]
+    print("r in hop", it1, it0, flush=True)
+    assert max(r.true_res for r in out["1"][1]) <= 1e-10
+    assert all(r.status == 1 for r in out["1"][1])
+    for a, b_ in zip(it1, it0):
+        assert abs(a - b_) <= 0.05 * b_ + 2
+    m = oracle.mobius_struct(L, ls, 1.8, 1.5, 0.5)
+    for k in range(nrhs):
+        ref, _, _ = oracle.mobius_invcg(u, src[k * n:(k + 1) * n], 0.02, 1e-12, 5000, m)
+        assert rel(out["1"][0][k * n:(k + 1) * n], ref) < 1e-8
 
 
 def test_invcg_device_matches_host(dwf4):
