@@ -46,8 +46,9 @@ hipError_t blas_cg_xr_mixed(int sx, const FieldRef& x, int sp, const FieldRef& p
 hipError_t blas_xpay_x_mixed(int sp, const FieldRef& p, int sr, const FieldRef& r, int sx, const FieldRef& x,
                              const double* alpha, const double* beta, const int* active, int64_t per_rhs, int nrhs,
                              int bs, hipStream_t s);
-// FP16 p, FP32 r and x, Moebius layout with Ls = 16: also t5 = b5 p + c5 P p
-hipError_t blas_xpay_x_s5(const FieldRef& p, const FieldRef& r, const FieldRef& x, const FieldRef& t5,
+// FP16 p, FP32 x, FP16 or FP32 r, Moebius layout with Ls = 16: also
+// t5 = b5 p + c5 P p
+hipError_t blas_xpay_x_s5(const FieldRef& p, int sr, const FieldRef& r, const FieldRef& x, const FieldRef& t5,
                           const double* alpha, const double* beta, const int* active, int64_t per_rhs, int nrhs, int bs,
                           double b5, double c5, double mass, hipStream_t s);
 hipError_t blas_cg_r_mixed(int sr, const FieldRef& r, int sa, const FieldRef& ap, const CGScalars& cg,

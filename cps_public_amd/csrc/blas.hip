@@ -239,6 +239,7 @@ __device__ __forceinline__ float ror_f(float v, int n) {
                 : __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x12F, 0xF, 0xF, true));
 }
 
+template <typename SR>
 __global__ void __launch_bounds__(256) k_xpay_x_s5(FieldRef p, FieldRef r, FieldRef x, FieldRef t5, const double* alpha,
                                                    const double* beta, const int* active, int64_t per_rhs, float b5,
                                                    float c5, float mass) {
@@ -264,7 +265,7 @@ __global__ void __launch_bounds__(256) k_xpay_x_s5(FieldRef p, FieldRef r, Field
   if (act) {
     const R b = R(beta[rhs]);
     cplx<R> vr[12];
-    spin_load<SpinF, R>(r, e, vr);
+    spin_load<SR, R>(r, e, vr);
     cplx<R> o[12], pr[12], t[12];
 #pragma unroll
     for (int c = 0; c < 12; c++) o[c] = cmk<R>(fma_r(b, R(vp[c].re), R(vr[c].re)), fma_r(b, R(vp[c].im), R(vr[c].im)));
@@ -518,18 +519,25 @@ hipError_t blas_xpay_x_mixed(int sp, const FieldRef& p, int sr, const FieldRef& 
   CPSHIP_XPX(STORE_H, SpinH, STORE_F, SpinF, STORE_F, SpinF)
   CPSHIP_XPX(STORE_H, SpinH, STORE_F, SpinF, STORE_H, SpinH)
   CPSHIP_XPX(STORE_H, SpinH, STORE_H, SpinH, STORE_H, SpinH)
+  CPSHIP_XPX(STORE_H, SpinH, STORE_H, SpinH, STORE_F, SpinF)
   CPSHIP_XPX(STORE_F, SpinF, STORE_F, SpinF, STORE_F, SpinF)
 #undef CPSHIP_XPX
   return hipErrorInvalidValue;
 }
 
-hipError_t blas_xpay_x_s5(const FieldRef& p, const FieldRef& r, const FieldRef& x, const FieldRef& t5,
+hipError_t blas_xpay_x_s5(const FieldRef& p, int sr, const FieldRef& r, const FieldRef& x, const FieldRef& t5,
                           const double* alpha, const double* beta, const int* active, int64_t per_rhs, int nrhs, int bs,
                           double b5, double c5, double mass, hipStream_t s) {
   if (per_rhs % 16 || bs % 64) return hipErrorInvalidValue;   // rows of 16 slots within whole wavefronts
   dim3 g = egrid(per_rhs, bs, nrhs), b(bs);
-  hipLaunchKernelGGL(k_xpay_x_s5, g, b, 0, s, p, r, x, t5, alpha, beta, active, per_rhs, float(b5), float(c5),
-                     float(mass));
+  if (sr == STORE_H)
+    hipLaunchKernelGGL(k_xpay_x_s5<SpinH>, g, b, 0, s, p, r, x, t5, alpha, beta, active, per_rhs, float(b5),
+                       float(c5), float(mass));
+  else if (sr == STORE_F)
+    hipLaunchKernelGGL(k_xpay_x_s5<SpinF>, g, b, 0, s, p, r, x, t5, alpha, beta, active, per_rhs, float(b5),
+                       float(c5), float(mass));
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

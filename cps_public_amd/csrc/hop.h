@@ -71,7 +71,7 @@ struct HopParams {
   // hop come from the neighbouring lanes of the site's 16-lane row (DPP
   // rotations) instead of loads and lane permutes
   int s5_row;
-  // rupd (with s5_row): out is not stored; instead the CG residual R (FP32)
+  // rupd (with s5_row): out is not stored; instead the CG residual R (FP16)
   // of every rhs with ractive[rhs] becomes R - ralpha[rhs] * out, and the
   // partials are |R|^2 (0 for the other rhs)
   int rupd;

@@ -1032,14 +1032,14 @@ __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem&
       if (P.ractive[rhs]) {
         const R a = R(P.ralpha[rhs]);
         cplx<R> rv[12];
-        spin_load<SpinF, R>(P.R, eo, rv);
+        spin_load<SpinH, R>(P.R, eo, rv);
 #pragma unroll
         for (int k = 0; k < 12; k++) {
           rv[k].re -= a * res[k].re;
           rv[k].im -= a * res[k].im;
           nrm += double(rv[k].re) * rv[k].re + double(rv[k].im) * rv[k].im;
         }
-        spin_store<SpinF, R>(P.R, eo, rv);
+        spin_store<SpinH, R>(P.R, eo, rv);
       }
     } else {
       spin_store<SOUT, R, NT>(P.out, eo, res);
