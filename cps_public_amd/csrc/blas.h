@@ -20,6 +20,8 @@ struct CGScalars {
   double delta2;      // mixed: reliable-update trigger |r_s|^2 < delta2 * rr_upd
   double* hist;       // optional |r|^2 history ring [nrhs][hist_len]
   int hist_len;
+  double* alpha_prev; // mixed CG with paired solution steps: alpha of the
+                      // iteration before (set by RED_CG_ALPHA), or nullptr
 };
 
 hipError_t blas_norm(int st, const FieldRef& x, int64_t per_rhs, int nrhs, int bs, double* partial, hipStream_t s);
@@ -51,6 +53,13 @@ hipError_t blas_xpay_x_mixed(int sp, const FieldRef& p, int sr, const FieldRef& 
 hipError_t blas_xpay_x_s5(const FieldRef& p, int sr, const FieldRef& r, const FieldRef& x, const FieldRef& t5,
                           const double* alpha, const double* beta, const int* active, int64_t per_rhs, int nrhs, int bs,
                           double b5, double c5, double mass, hipStream_t s);
+// paired iterations (FP16 p in two buffers, FP32 x): p_out = r + beta p_in;
+// with xupd first x += a_old p_out(before) + a_new p_in; t5 (optional,
+// Moebius Ls 16): b5 p_out + c5 P p_out
+hipError_t blas_xpay_pair(const FieldRef& pin, const FieldRef& pout, int sr, const FieldRef& r, const FieldRef& x,
+                          const FieldRef* t5, bool xupd, const double* a_old, const double* a_new, const double* beta,
+                          const int* active, int64_t per_rhs, int nrhs, int bs, double b5, double c5, double mass,
+                          hipStream_t s);
 hipError_t blas_cg_r_mixed(int sr, const FieldRef& r, int sa, const FieldRef& ap, const CGScalars& cg,
                            int64_t per_rhs, int nrhs, int bs, double* partial, hipStream_t s);
 hipError_t blas_xpay_mixed(int sp, const FieldRef& p, int sr, const FieldRef& r, const double* beta, const int* active,

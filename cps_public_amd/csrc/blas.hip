@@ -285,6 +285,86 @@ __global__ void __launch_bounds__(256) k_xpay_x_s5(FieldRef p, FieldRef r, Field
   }
 }
 
+// Paired iterations of the mixed CG with FP16 directions in two buffers: an
+// iteration writes its new direction p_out = r + beta p_in into the buffer
+// that held the direction before p_in; with XUPD (every second iteration)
+// the two pending solution steps are applied first, in order,
+// x += a_old p_out(before) ; x += a_new p_in -- the same fused multiply-adds
+// as one step per iteration, with one pass over x for two iterations.  S5:
+// also t5 = b5 p_out + c5 P p_out (as k_xpay_x_s5).  A rhs that stopped in
+// the first iteration of a pair gets its last direction moved into p_out.
+template <typename SR, bool XUPD, bool S5>
+__global__ void __launch_bounds__(256) k_xpay_pair(FieldRef pin, FieldRef pout, FieldRef r, FieldRef x, FieldRef t5,
+                                                   const double* a_old, const double* a_new, const double* beta,
+                                                   const int* active, int64_t per_rhs, float b5, float c5,
+                                                   float mass) {
+  const int rhs = blockIdx.y;
+  const double ao = XUPD ? a_old[rhs] : 0.0, an = XUPD ? a_new[rhs] : 0.0;
+  const bool act = active[rhs] != 0;
+  if (!act && ao == 0.0 && an == 0.0) return;
+  const int64_t e = (int64_t)rhs * per_rhs + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  using R = float;
+  cplx<R> vi[12];
+  spin_load<SpinH, R>(pin, e, vi);
+  if (XUPD && (ao != 0.0 || an != 0.0)) {
+    cplx<R> vx[12];
+    spin_load<SpinF, R>(x, e, vx);
+    if (ao != 0.0) {
+      const R a = R(ao);
+      cplx<R> vo[12];
+      spin_load<SpinH, R>(pout, e, vo);
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+        vx[c].re = fma_r(a, vo[c].re, vx[c].re);
+        vx[c].im = fma_r(a, vo[c].im, vx[c].im);
+      }
+    }
+    if (an != 0.0) {
+      const R a = R(an);
+#pragma unroll
+      for (int c = 0; c < 12; c++) {
+        vx[c].re = fma_r(a, vi[c].re, vx[c].re);
+        vx[c].im = fma_r(a, vi[c].im, vx[c].im);
+      }
+    }
+    spin_store<SpinF, R>(x, e, vx);
+  }
+  if (XUPD && !act && an != 0.0) {
+    // the rhs stopped at the previous iteration (its step a_new is pending):
+    // its latest direction p_in is copied, unchanged, into the buffer that
+    // holds the direction at the end of a pair
+    const uint4* src = reinterpret_cast<const uint4*>(pin.data) + e;
+    uint4* dst = reinterpret_cast<uint4*>(pout.data) + e;
+#pragma unroll
+    for (int k = 0; k < 3; k++) dst[k * pout.plane] = src[k * pin.plane];
+    pout.norm[e] = pin.norm[e];
+  }
+  if (act) {
+    const R b = R(beta[rhs]);
+    cplx<R> vr[12];
+    spin_load<SR, R>(r, e, vr);
+    cplx<R> o[12];
+#pragma unroll
+    for (int c = 0; c < 12; c++) o[c] = cmk<R>(fma_r(b, vi[c].re, R(vr[c].re)), fma_r(b, vi[c].im, R(vr[c].im)));
+    spin_store<SpinH, R>(pout, e, o);
+    if constexpr (S5) {
+      cplx<R> pr[12], t[12];
+      half_roundtrip(o, pr);
+      const int s = (int)(e & 15);
+      const R cm = (s == 0) ? -mass : R(1), cp = (s == 15) ? -mass : R(1);
+      const R one = R(1);
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        t[k] = cmk<R>(one * fma_r(b5, pr[k].re, c5 * (cm * ror_f(pr[k].re, 1))),
+                      one * fma_r(b5, pr[k].im, c5 * (cm * ror_f(pr[k].im, 1))));
+        t[6 + k] = cmk<R>(one * fma_r(b5, pr[6 + k].re, c5 * (cp * ror_f(pr[6 + k].re, 15))),
+                          one * fma_r(b5, pr[6 + k].im, c5 * (cp * ror_f(pr[6 + k].im, 15))));
+      }
+      spin_store<SpinH, R>(t5, e, t);
+    }
+  }
+}
+
 // r -= alpha A p and the per-block |r|^2 (the solution step is lagged)
 template <typename SR, typename SA>
 __global__ void __launch_bounds__(256) k_cg_r_t(FieldRef r, FieldRef ap, const double* alpha, const int* active,
@@ -356,6 +436,7 @@ __global__ void k_reduce_rhs(const double* partial, int bpr, double* out, int mo
   if (mode == RED_CG_ALPHA) {
     // d = <p, A p>  ->  a = rr / d
     cg.d[rhs] = v;
+    if (cg.alpha_prev) cg.alpha_prev[rhs] = cg.alpha[rhs];
     if (cg.active[rhs]) {
       cg.iters[rhs] += 1;
       cg.rr_prev[rhs] = cg.rr[rhs];
@@ -538,6 +619,30 @@ hipError_t blas_xpay_x_s5(const FieldRef& p, int sr, const FieldRef& r, const Fi
                        float(c5), float(mass));
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t blas_xpay_pair(const FieldRef& pin, const FieldRef& pout, int sr, const FieldRef& r, const FieldRef& x,
+                          const FieldRef* t5, bool xupd, const double* a_old, const double* a_new, const double* beta,
+                          const int* active, int64_t per_rhs, int nrhs, int bs, double b5, double c5, double mass,
+                          hipStream_t s) {
+  if (t5 && (per_rhs % 16 || bs % 64)) return hipErrorInvalidValue;   // rows of 16 slots in whole wavefronts
+  dim3 g = egrid(per_rhs, bs, nrhs), b(bs);
+  const FieldRef tt = t5 ? *t5 : pout;
+  const float fb = float(b5), fc = float(c5), fm = float(mass);
+#define CPSHIP_PAIR(TR, XU, S5)                                                                               \
+  hipLaunchKernelGGL((k_xpay_pair<TR, XU, S5>), g, b, 0, s, pin, pout, r, x, tt, a_old, a_new, beta, active,  \
+                     per_rhs, fb, fc, fm)
+  if (sr == STORE_H) {
+    if (t5) { if (xupd) CPSHIP_PAIR(SpinH, true, true); else CPSHIP_PAIR(SpinH, false, true); }
+    else { if (xupd) CPSHIP_PAIR(SpinH, true, false); else CPSHIP_PAIR(SpinH, false, false); }
+  } else if (sr == STORE_F) {
+    if (t5) { if (xupd) CPSHIP_PAIR(SpinF, true, true); else CPSHIP_PAIR(SpinF, false, true); }
+    else { if (xupd) CPSHIP_PAIR(SpinF, true, false); else CPSHIP_PAIR(SpinF, false, false); }
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef CPSHIP_PAIR
   return hipGetLastError();
 }
 

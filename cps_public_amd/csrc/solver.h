@@ -35,7 +35,7 @@ FieldRef null_field();
 
 enum Slot {
   S_X = 0, S_SRC, S_R, S_P, S_AP, S_T1, S_T2, S_XS, S_RS, S_PS, S_APS, S_T1S, S_T2S, S_T3, S_T3S,
-  S_IO_IN, S_IO_OUT, S_EVEN_IN, S_EVEN_OUT, S_TMP, S_BENCH_IN, S_BENCH_OUT, S_NSLOTS
+  S_IO_IN, S_IO_OUT, S_EVEN_IN, S_EVEN_OUT, S_TMP, S_BENCH_IN, S_BENCH_OUT, S_PS2, S_NSLOTS
 };
 
 struct OpLayout {
@@ -172,6 +172,7 @@ struct CGState {
   CGScalars s;
   double* rr_true = nullptr;    // FP64 |r|^2 per rhs (true residual evaluations)
   double* upd_store = nullptr;  // backing store of s.rr_upd
+  double* alpha_prev_store = nullptr;  // backing store of s.alpha_prev
   int* mask = nullptr;          // per-rhs flags for masked BLAS launches
   int alloc(Lattice& L, int nrhs, int hist_len);
 };

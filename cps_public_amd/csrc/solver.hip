@@ -546,7 +546,7 @@ int download(Lattice& L, const Field& f, double* host, const OpLayout& o, int cb
 
 int CGState::alloc(Lattice& L, int nrhs_, int hist_len_) {
   nrhs = nrhs_;
-  const size_t nd = 10, ni = 4;
+  const size_t nd = 11, ni = 4;
   const size_t hist = (size_t)nrhs * hist_len_;
   int rc = L.scal.ensure((nd * nrhs + hist) * sizeof(double) + ni * nrhs * sizeof(int) + 256);
   if (rc) return rc;
@@ -562,6 +562,8 @@ int CGState::alloc(Lattice& L, int nrhs_, int hist_len_) {
   rr_true = d + 8 * nrhs;
   s.rr_upd = nullptr;          // set by the mixed solver
   upd_store = d + 9 * nrhs;
+  alpha_prev_store = d + 10 * nrhs;
+  s.alpha_prev = nullptr;      // set by the mixed solver
   s.delta2 = 0.0;
   s.hist = hist_len_ ? d + nd * nrhs : nullptr;
   s.hist_len = hist_len_ ? hist_len_ : 1;
@@ -666,6 +668,7 @@ static CGScalars scalars_view(const CGScalars& s, int lo) {
   v.iters += lo;
   v.status += lo;
   if (v.rr_upd) v.rr_upd += lo;
+  if (v.alpha_prev) v.alpha_prev += lo;
   if (v.hist) v.hist += (int64_t)lo * v.hist_len;
   return v;
 }
@@ -866,6 +869,7 @@ __global__ void k_mixed_after_update(CGScalars cg, const double* rr_true, const 
   // rp aliases alpha: read it, then leave no solution step pending
   const double rpk = rp[k];
   cg.alpha[k] = 0.0;
+  if (cg.alpha_prev) cg.alpha_prev[k] = 0.0;
   coef[k] = 0.0;
   const int st = cg.status[k];
   if (st == CG_STATUS_D_ZERO || st == CG_STATUS_MAXITER) return;
@@ -975,6 +979,24 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   }();
   Field& t3s = L.field_min(S_T3S, sps, n);
   if (L.last_rc) return L.last_rc;
+  // FP16 directions and an FP32 solution: iterations run in pairs with the
+  // direction in two buffers, and the solution is updated every second
+  // iteration with both pending steps, in order (bit-identical to one step
+  // per iteration, one pass over x less per pair; CPS_HIP_PAIRED_X=0 turns
+  // it off)
+  const bool paired = [&] {
+    const char* e = getenv("CPS_HIP_PAIRED_X");
+    return lagged && sps == STORE_H && xvec == STORE_F && !(e && *e == '0');
+  }();
+  Field* ps2 = nullptr;
+  if (paired) {
+    ps2 = &L.field(S_PS2, sps, n);
+    if (L.last_rc) return L.last_rc;
+    cg.s.alpha_prev = cg.alpha_prev_store;
+    std::vector<double> zeros(nrhs, 0.0);
+    CPSHIP_CHECK(hipMemcpyAsync(cg.s.alpha_prev, zeros.data(), nrhs * sizeof(double), hipMemcpyHostToDevice,
+                                L.stream));
+  }
   DevBuf coef_buf;
   if ((rc = coef_buf.ensure(nrhs * sizeof(double)))) return rc;
   double* coef = (double*)coef_buf.p;
@@ -1004,7 +1026,46 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     const FieldRef xsw = rhs_view(xs.ref(), o.per_rhs, lo), rsw = rhs_view(rs.ref(), o.per_rhs, lo);
     const FieldRef psw = rhs_view(ps.ref(), o.per_rhs, lo), apsw = rhs_view(aps.ref(), o.per_rhs, lo);
     const CGScalars sw = scalars_view(cg.s, lo);
-    const int todo = std::min(std::min(chunk, max_itr - done), every - since);
+    int todo = std::min(std::min(chunk, max_itr - done), every - since);
+    // the operator on this iteration's direction pw, alpha, the residual step
+    // (in the last hop of M^dag, or its own pass) and beta
+    auto opstep = [&](const FieldRef& pw) -> int {
+      if (r_in_hop) {
+        // M p with |M p|^2, alpha, then r -= alpha M^dag (M p) with |r|^2
+        int e = mobius_m(L, op, hprec, t3s.ref(), pw, t1s.ref(), t2s.ref(), 0, part, nw, true);
+        if (e) return e;
+        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
+        e = mobius_mdag_rupdate(L, op, hprec, rsw, t3s.ref(), t1s.ref(), t2s.ref(), sw.alpha, sw.active, part, nw);
+        if (e) return e;
+        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_BETA, sw, L.stream));
+        return 0;
+      }
+      int e = apply_mdagm(L, op, hprec, apsw, pw, t1s.ref(), t2s.ref(), part, nw, s5_in_xpay);
+      if (e) return e;
+      CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
+      CPSHIP_CHECK(blas_cg_r_mixed(svec, rsw, sps, apsw, sw, o.per_rhs, nw, bsb, part, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, nullptr, RED_CG_BETA, sw, L.stream));
+      return 0;
+    };
+    if (paired) {
+      // iteration a: p_B = r + beta p_A ; iteration b: x += alpha_prev p_A +
+      // alpha p_B, p_A = r + beta p_B (chunks are whole pairs, so the latest
+      // direction is in A at every chunk boundary)
+      const FieldRef pbw = rhs_view(ps2->ref(), o.per_rhs, lo);
+      const FieldRef t1ref = t1s.ref();
+      const FieldRef* t5 = s5_in_xpay ? &t1ref : nullptr;
+      const int pairs = (todo + 1) / 2;
+      todo = 2 * pairs;
+      rc = iter_graph.run(L, pairs, [&]() -> int {
+        CPSHIP_CHECK(blas_xpay_pair(psw, pbw, svec, rsw, xsw, t5, false, nullptr, nullptr, sw.beta, sw.active,
+                                    o.per_rhs, nw, bsb, op->mobius_b, op->mobius_c, op->mass, L.stream));
+        int e = opstep(pbw);
+        if (e) return e;
+        CPSHIP_CHECK(blas_xpay_pair(pbw, psw, svec, rsw, xsw, t5, true, sw.alpha_prev, sw.alpha, sw.beta,
+                                    sw.active, o.per_rhs, nw, bsb, op->mobius_b, op->mobius_c, op->mass, L.stream));
+        return opstep(psw);
+      });
+    } else
     // one iteration: the previous step's direction update p = r + beta p
     // first, then Ap, alpha, x and r, |r|^2 and beta -- so that a reliable
     // update at a chunk boundary falls between r's update and p's

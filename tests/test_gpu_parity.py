@@ -246,6 +246,39 @@ def test_mixed_lagged_solution_step_bit_identical(monkeypatch, kind):
     assert max(r.true_res for r in out["1"][1]) <= 1e-10
 
 
+@pytest.mark.parametrize("kind,check_every", [("dwf", 0), ("mobius16", 0), ("dwf", 7)])
+def test_mixed_paired_solution_steps_bit_identical(monkeypatch, kind, check_every):
+    """Iterations in pairs with the direction in two buffers and the solution
+    updated every second iteration (both pending steps, in order) give the
+    same bits and iteration counts as one step per iteration; an odd check
+    interval is rounded up to whole pairs."""
+    if kind == "dwf":
+        L, ls = [4, 4, 4, 8], 8
+        op = C.make_op(C.OP_DWF, mass=0.01, m5=1.8)
+    else:
+        L, ls = [4, 4, 4, 8], 16
+        op = C.make_op(C.OP_MOBIUS, mass=0.02, m5=1.8, b=1.5, c=0.5)
+    u = wilson_gauge(L, 0.2, 310)
+    vcb = int(np.prod(L)) // 2
+    src = np.concatenate([F.gaussian_fermion(vcb * ls, 311 + k) * (1.0 + 5.0 * k) for k in range(3)])
+    arg = C.make_cg_arg(1e-10, 5000, precision=C.PREC_MIXED_HALF, reliable_every=40)
+    arg.check_every = check_every
+    out = {}
+    for paired in ("1", "0"):
+        monkeypatch.setenv("CPS_HIP_PAIRED_X", paired)
+        lat = C.Lattice(L, ls, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        out[paired] = lat.invcg(op, src, arg, nrhs=3)
+        lat.close()
+    it = [r.iterations for r in out["1"][1]]
+    print("paired", kind, check_every, it, [r.iterations for r in out["0"][1]], flush=True)
+    if check_every == 0:
+        assert it == [r.iterations for r in out["0"][1]]
+        assert np.array_equal(out["1"][0], out["0"][0])
+    assert max(r.true_res for r in out["1"][1]) <= 1e-10
+    assert all(r.status == 1 for r in out["1"][1])
+
+
 def test_mixed_s5_in_direction_update_bit_identical(monkeypatch):
     """Moebius, Ls 16, FP16 directions: the direction update writing the
     first step of M (b p + c P p, slots s-1, s+1 from the neighbouring lanes)
