@@ -257,6 +257,8 @@ def main():
                          "default run within minutes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
+    ap.add_argument("--no-dirac-res", action="store_true",
+                    help="skip the untimed solve that reports |xi - M_pc x| / |xi| for rhs 0")
     ap.add_argument("--hop-reps", type=int, default=50)
     ap.add_argument("--max-iter", type=int, default=20000,
                     help="CG iteration cap (a small cap gives a bounded run for kernel traces)")
@@ -314,6 +316,19 @@ def main():
                 "true_res_max": max(r.true_res for r in res64)}
     gpu_fp64_iters = (fp64["iterations"][0] if fp64 else res_main[0].iterations)
 
+    # residual of the Dirac equation M_pc x = xi itself (not of the normal
+    # equations the CG works on) for rhs 0, untimed: the same solver on the
+    # source M_pc^dag xi, then |xi - M_pc x| / |xi| in FP64
+    dirac_res = None
+    if not a.no_dirac_res:
+        per = src.size // nrhs
+        xi = np.ascontiguousarray(src[:per])
+        x1, r1 = lat.invcg(op, lat.matpc(op, xi, dag=1), arg_main, nrhs=1)
+        rd = xi - lat.matpc(op, x1, dag=0)
+        dirac_res = {"rhs": 0, "res": float(np.linalg.norm(rd) / np.linalg.norm(xi)),
+                     "iterations": r1[0].iterations, "normal_eq_true_res": r1[0].true_res}
+        print(f"# Dirac-equation residual (rhs 0): {dirac_res['res']:.3e}", file=sys.stderr, flush=True)
+
     # dominant kernel: the hopping kernel of the headline path, timed with HIP
     # events on its launch stream
     hk = cfg["kind"] if cfg["kind"] != C.OP_MOBIUS else C.OP_MOBIUS
@@ -353,6 +368,7 @@ def main():
                                3: "f32 spinors + f32 links, f64 reliable updates"}[prec_main]},
         "iterations": [r.iterations for r in res_main],
         "true_res_max": max(r.true_res for r in res_main),
+        "dirac_eq_res": dirac_res,
         "reliable_updates": [r.reliable_updates for r in res_main],
         "fp64": fp64,
         "speedup_vs_fp64": (fp64["s_per_rhs"] / s_per_rhs) if fp64 else None,
