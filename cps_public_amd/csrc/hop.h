@@ -56,7 +56,12 @@ struct HopParams {
   double xcoef;
   FieldRef in, out, X;
   GaugeRef gauge;
-  double* partial;   // per-block |out|^2, or nullptr
+  double* partial;   // per-block |out|^2 (per-chunk with chunk_part), or nullptr
+  // chunk_part (set by the launcher where hop_chunk_partials holds): one
+  // partial per 64 consecutive elements of a rhs, summed as a pairwise tree
+  // in element order (the same bits in every thread mapping that computes
+  // it), at partial[rhs * (elements per rhs / 64) + chunk]
+  int chunk_part;
   // clover epilogue (Wilson kind only): 0 none, CLOV_POST out = A (scale*hop),
   // CLOV_X out = xcoef*(A X) + kscale[set]*scale*hop, with A the clover set of
   // kappa index set = rhs / rhs_per_set at the output checkerboard and kscale
@@ -138,11 +143,24 @@ constexpr int LDS_SITE_WORDS = 68;
 inline int lds_sites_per_block(int bs, int nb) { return bs % nb == 0 ? bs / nb : (bs - 1) / nb + 2; }
 inline size_t lds_link_bytes(int sites) { return (size_t)sites * (LDS_SITE_WORDS + 8) * 4; }
 
+// Shapes whose |out|^2 partials are per 64-element chunk rather than per
+// block: Wilson kind with one element per site (the 4-D Wilson and clover
+// operators) and a whole number of chunks per rhs
+inline bool chunk_partials_shape(int kind, int nb, long long per_rhs) {
+  return kind == KIND_WILSON && nb == 1 && per_rhs % 64 == 0;
+}
+inline bool hop_chunk_partials(const HopParams& P) {
+  return P.partial && chunk_partials_shape(P.kind, P.nb, (long long)P.nsites_out * P.nb);
+}
+
 // HOP_RHSGROUP: whether the launch shape allows it at block size bs; sets
-// the rhs-per-wavefront shift
+// the rhs-per-wavefront shift.  With partials only in the chunk form, and
+// with G <= the wavefronts of a block, so that a chunk's G wavefronts (S =
+// 64 / G sites each) lie in one block
 inline bool rhsgroup_setup(HopParams& Q, int bs) {
   const int G = hop_rhsgroup_G(Q.flags);
-  if (Q.kind != KIND_WILSON || Q.partial || (Q.flags & (HOP_SITEWAVE | HOP_LDS_LINKS | HOP_SPATIAL))) return false;
+  if (Q.partial && !(hop_chunk_partials(Q) && bs % 64 == 0 && G <= bs / 64)) return false;
+  if (Q.kind != KIND_WILSON || (Q.flags & (HOP_SITEWAVE | HOP_LDS_LINKS | HOP_SPATIAL))) return false;
   if (Q.nb < 1 || (Q.nb & (Q.nb - 1)) || G * Q.nb > 64 || Q.nrhs % G || bs < 64 || bs % 64) return false;
   const int S = 64 / (G * Q.nb);
   if (Q.nsites_out % ((bs / 64) * S)) return false;

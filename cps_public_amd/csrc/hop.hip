@@ -16,6 +16,10 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
   Q.bpr = (int)(per_rhs / bs);
   Q.bpr_b = (int)(((int64_t)P.perm_S * P.nb) / bs);
   Q.nb_shift = (P.nb & (P.nb - 1)) == 0 ? __builtin_ctz((unsigned)P.nb) : -1;
+  // partials per 64-site chunk: wavefront-aligned blocks, and a mapping that
+  // keeps a chunk's sites in one block (element order or the rhs groups)
+  Q.chunk_part = hop_chunk_partials(P) ? 1 : 0;
+  if (Q.chunk_part && (bs % 64 || (P.flags & (HOP_SPATIAL | HOP_SITEWAVE)))) return hipErrorInvalidValue;
   const bool nt = (P.flags & HOP_NT_STORE) != 0;
   // HOP_LDS_LINKS: half-precision spinors, one link per thread to stage
   const bool lds = (P.flags & HOP_LDS_LINKS) != 0;

@@ -523,6 +523,41 @@ __device__ __forceinline__ void block_partial(R v, double* partial, int64_t slot
   __syncthreads();   // red[] is reused by the block's next sub-block
 }
 
+// Chunk partials (HopParams::chunk_part, one element per site): the sum of
+// v over 64 consecutive sites of one rhs as the pairwise tree of the xor
+// butterfly (offsets 1, 2, .., 32; lane i and lane i^m form the same sum).
+// Element order: a wavefront is one chunk.
+__device__ __forceinline__ void chunk_partial_elem(const HopParams& P, int rhs, int site, double v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m);
+  if ((threadIdx.x & 63) == 0) P.partial[(int64_t)rhs * (P.nsites_out >> 6) + (site >> 6)] = v;
+}
+
+// HOP_RHSGROUP (lane = g S + q, S = 64 / G sites of rhs g): the butterfly
+// over the S lanes of each rhs gives the first log2 S levels of that tree;
+// the G wavefronts of a chunk (in one block, G <= W) then combine pairwise,
+// (a0 + a1) + (a2 + a3), as the remaining levels.
+__device__ __forceinline__ void chunk_partial_rhsgroup(const HopParams& P, int64_t sb64, double v) {
+  __shared__ double cred[16];
+  const int s_shift = 6 - P.rg_shift, S = 1 << s_shift, G = 1 << P.rg_shift;
+  for (int m = 1; m < S; m <<= 1) v += __shfl_xor(v, m);
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6), W = (int)(blockDim.x >> 6);
+  if ((lane & (S - 1)) == 0) cred[w * G + (lane >> s_shift)] = v;
+  __syncthreads();
+  const int t = (int)threadIdx.x;
+  if (t < W) {
+    const int c = t >> P.rg_shift, g = t & (G - 1);   // chunk c of the block, rhs g of the group
+    // G is 2 or 4 (G <= W <= 4)
+    const double* cw = cred + c * G * G + g;
+    const double sum = G == 2 ? cw[0] + cw[G] : (cw[0] + cw[G]) + (cw[2 * G] + cw[3 * G]);
+    const int sb = (int)sb64;
+    const int per_g = P.nsites_out / (W << s_shift);
+    const int rg = sb / per_g, sblk = sb - rg * per_g;
+    const int64_t rhs = ((int64_t)rg << P.rg_shift) + g;
+    P.partial[rhs * (P.nsites_out >> 6) + (int64_t)sblk * (W >> P.rg_shift) + c] = sum;
+  }
+}
+
 // Maps the hardware block id so that each XCD (blocks are dealt round-robin
 // over the 8 XCDs) works on a contiguous range of sites, which keeps
 // neighbour reuse inside one XCD's L2.
@@ -1068,7 +1103,11 @@ __global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) 
   const Nbr N = hop_nbr<KIND>(P, E);
   const double nrm = hop_element<SIN, SOUT, LK, R, DAG, KIND, NT, CLOV, SW == 2>(
       P, E, N, reinterpret_cast<uint32_t*>(hop_lds_area));
-  if (P.partial) block_partial<double>(nrm, P.partial, E.pslot);
+  if (P.partial) {
+    if (!P.chunk_part) block_partial<double>(nrm, P.partial, E.pslot);
+    else if constexpr (SW == 3) chunk_partial_rhsgroup(P, hb, nrm);
+    else chunk_partial_elem(P, E.rhs, E.site, nrm);
+  }
 }
 
 }  // namespace cpship

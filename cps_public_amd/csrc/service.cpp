@@ -14,7 +14,7 @@
 //
 // A right-hand side's arithmetic never depends on the other slots: the hop
 // and BLAS kernels are element-wise, and every reduction sums one slot's
-// block partials in a fixed order.  A solve therefore gives the same bits
+// block (or 64-site chunk) partials in a fixed order.  A solve therefore gives the same bits
 // alone and in any mix.
 #include <algorithm>
 #include <chrono>

@@ -294,8 +294,9 @@ int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, con
 // Autotuner: the first launch of each hop shape (kind, precision, dag, batch,
 // volume, rhs count, epilogue) times every candidate block size and launch
 // variant on the real operands and keeps the fastest for the lattice
-// handle's lifetime.  Launches that write reduction partials keep the
-// default block size, because the reduction reads one partial per block.
+// handle's lifetime.  Launches that write per-block reduction partials keep
+// the default block size, because the reduction reads one partial per block;
+// per-chunk partials (4-D operators) do not depend on the block or mapping.
 // CPS_HIP_TUNE=0 switches tuning off (default block, no variants).
 
 static bool tuning_enabled() {
@@ -397,17 +398,19 @@ int tuned_launch(Lattice& L, HopParams& P) {
     }
   }
   if (!have) {
-    // launches that write reduction partials use a fixed order (results do
-    // not depend on tuning): spatial for the 5-D layouts (nb >= 8 slots per
-    // site) whenever the geometry allows it, element order otherwise (with
-    // one site per lane the spatial sweep is the slower order: 6.2 against
-    // 4.6 ms for the 32^3x64 clover hop with 48 rhs)
+    // launches that write per-block reduction partials use a fixed order
+    // (results do not depend on tuning): spatial for the 5-D layouts (nb >= 8
+    // slots per site) whenever the geometry allows it, element order
+    // otherwise (with one site per lane the spatial sweep is the slower
+    // order: 6.2 against 4.6 ms for the 32^3x64 clover hop with 48 rhs); with
+    // per-chunk partials the element order is the default and the rhs-group
+    // mapping a candidate
     HopTuneVal best{def_block, (P.nb >= 8 && spatial_ok(L, P, def_block)) ? HOP_SPATIAL : 0, 0.0};
     // a launch that updates a field in place (rupd) is never timed on the
     // live data: it keeps the fixed order of launches with partials
     if (tuning_enabled() && !P.rupd) {
       std::vector<int> blocks;
-      if (P.partial) blocks.push_back(def_block);
+      if (P.partial && !hop_chunk_partials(P)) blocks.push_back(def_block);
       else
         for (int b : {64, 128, 256})
           if (b <= def_block && per_rhs % b == 0) blocks.push_back(b);
@@ -425,7 +428,7 @@ int tuned_launch(Lattice& L, HopParams& P) {
         for (int f : cand) {
           if (f & HOP_RHSGROUP) {
             if (!rhsgroup_ok(P, b, f) || mode == 1 || mode == 2 || mode == 3) continue;
-          } else if (mode == 4 && rhsgroup_ok(P, b, HOP_RHSGROUP)) {
+          } else if (mode == 4 && (rhsgroup_ok(P, b, HOP_RHSGROUP) || rhsgroup_ok(P, def_block, HOP_RHSGROUP))) {
             continue;   // CPS_HIP_HOP_MODE=rhsgroup
           }
           if (f & HOP_SPATIAL) {
@@ -478,6 +481,7 @@ int hop_blocks_per_rhs(Lattice& L, const cps_hip_op* op) {
   OpLayout o;
   if (op_layout(op, L.ls, &o, L.g)) return 1;
   const long long n = (long long)o.nsites * o.nb;
+  if (chunk_partials_shape(op->kind == CPS_HIP_OP_DWF ? KIND_DWF : KIND_WILSON, o.nb, n)) return (int)(n / 64);
   return (int)(n / pick_block(n));
 }
 

@@ -941,10 +941,45 @@ def test_rhsgroup_mapping_bit_identical(monkeypatch, case):
         else:
             op = C.make_op(C.OP_WILSON, mass=0.1)
         res[mode] = (lat.matpc(op, src, 0, nrhs=nrhs), lat.matpc(op, src, 1, nrhs=nrhs),
-                     lat.mdagm_prec(op, src, C.PREC_MIXED_HALF, nrhs=nrhs)[0])
+                     *lat.mdagm_prec(op, src, C.PREC_MIXED_HALF, nrhs=nrhs), *lat.mdagm(op, src, nrhs=nrhs))
         lat.close()
     for a, b_ in zip(res["element"], res["rhsgroup"]):
         assert np.array_equal(a, b_)
+
+
+@pytest.mark.parametrize("case", ["clover", "wilson"])
+def test_mixed_cg_mapping_consistent(monkeypatch, case):
+    """A mixed-precision CG with the 4-D operators' hops free to use the
+    element or the rhs-group mapping (the launch with the |M p|^2 chunk
+    partials included) takes the same iterations to the same solution.  The
+    operators and the partial sums are bit-identical between the mappings
+    (test_rhsgroup_mapping_bit_identical); the clover solve with two kappa
+    sets still takes a different rounding path for the second set (measured
+    1.8e-12 apart; the same with per-block partials, where the launch with
+    partials never used the rhs-group mapping), so the solutions are
+    compared to 1e-9, well inside what a 1e-10 residual bound allows."""
+    L = [4, 4, 4, 8]
+    u = wilson_gauge(L, 0.2, 260)
+    nrhs = 4
+    vcb = int(np.prod(L)) // 2
+    src = np.concatenate([F.gaussian_fermion(vcb, 261 + k) for k in range(nrhs)])
+    arg = C.make_cg_arg(1e-10, 2000, precision=C.PREC_MIXED_HALF, reliable_every=50)
+    res = {}
+    for mode in ("element", "rhsgroup"):
+        monkeypatch.setenv("CPS_HIP_HOP_MODE", mode)
+        lat = C.Lattice(L, 1, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        if case == "clover":
+            lat.clover_build(CLOVER_KAPPAS, CLOVER_CSW)
+            op = clover_op(CLOVER_KAPPAS[0])
+        else:
+            op = C.make_op(C.OP_WILSON, mass=0.1)
+        x, r = lat.invcg(op, src, arg, nrhs=nrhs)
+        res[mode] = (x, [k.iterations for k in r], [k.true_res for k in r])
+        lat.close()
+    assert res["element"][1] == res["rhsgroup"][1]
+    assert max(res["element"][2] + res["rhsgroup"][2]) < 1e-10
+    assert rel(res["rhsgroup"][0], res["element"][0]) < 1e-9
 
 
 @pytest.mark.parametrize("kind,ls,nrhs", [("mobius", 16, 2), ("mobius", 12, 1), ("mobius", 8, 1), ("dwf", 16, 2)])
