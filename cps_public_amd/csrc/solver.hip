@@ -380,8 +380,10 @@ int tuned_launch(Lattice& L, HopParams& P) {
     } else {
       // the same shape tuned for another rhs count (a CG's active-rhs window
       // shrinking): reuse that choice where it is valid for this count (for
-      // launches with partials it is the fixed-order choice, which does not
-      // depend on the count)
+      // launches with per-block partials it is the fixed-order choice, which
+      // does not depend on the count; chunk partials do not depend on the
+      // mapping, so a rhs-group choice that the count no longer allows falls
+      // back to element order with the same bits)
       for (const auto& kv : tune_cache) {
         HopTuneKey k2 = kv.first;
         k2.nrhs = key.nrhs;
