@@ -350,8 +350,9 @@ int clover_export(Lattice& L, int k, int cb, double* A) {
 // MatPc (dag 0) / MatPcDag (dag 1): out = A_oo in - kappa^2 D A_ee^-1 D in,
 // with D the hop of the given dag (A is Hermitian).  partial: per-block
 // |out|^2 of the second hop.
-int clover_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
-                 const FieldRef& t1, int dag, double* partial, int nrhs) {
+static int clover_matpc_ru(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                           const FieldRef& t1, int dag, double* partial, int nrhs, const FieldRef* R,
+                           const double* ralpha, const int* ractive) {
   int per = 0;
   int rc = clover_sets(L, op, nrhs, &per);
   if (rc) return rc;
@@ -368,7 +369,24 @@ int clover_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& ou
   if (rc) return rc;
   hc.mode = CLOV_X;
   hc.kscale = (const double*)L.clover_ksq.p;
+  hc.R = R;
+  hc.ralpha = ralpha;
+  hc.ractive = ractive;
   return hop_ex(L, op, hprec, out, t1, 1, dag, -1.0, in, 1.0, partial, nrhs, &hc);
+}
+
+int clover_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                 const FieldRef& t1, int dag, double* partial, int nrhs) {
+  return clover_matpc_ru(L, op, hprec, out, in, t1, dag, partial, nrhs, nullptr, nullptr, nullptr);
+}
+
+// MatPcDag of t2 with the CG residual step in the epilogue of its last hop:
+// r -= alpha[rhs] (MatPcDag t2) for the rhs with active[rhs], partial =
+// |r|^2 (FP16 r and hops; M^dag M p is never stored)
+int clover_mdag_rupdate(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& r, const FieldRef& t2,
+                        const FieldRef& t1, const double* alpha, const int* active, double* partial, int nrhs) {
+  if (storage_of_hprec(hprec) != CLOVER_H) return CPS_HIP_ERR_ARG;
+  return clover_matpc_ru(L, op, hprec, r, t2, t1, 1, partial, nrhs, &r, alpha, active);
 }
 
 // MatPcDagMatPc (d_op_clover.C:156-181): partial = |MatPc in|^2 per block.

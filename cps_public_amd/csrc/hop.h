@@ -76,7 +76,8 @@ struct HopParams {
   // hop come from the neighbouring lanes of the site's 16-lane row (DPP
   // rotations) instead of loads and lane permutes
   int s5_row;
-  // rupd (with s5_row): out is not stored; instead the CG residual R (FP16)
+  // rupd (with s5_row, or with the CLOV_X epilogue of FP16 hops): out is
+  // not stored; instead the CG residual R (FP16)
   // of every rhs with ractive[rhs] becomes R - ralpha[rhs] * out, and the
   // partials are |R|^2 (0 for the other rhs)
   int rupd;
@@ -113,9 +114,10 @@ struct HopParams {
 };
 
 // CLOV_S5XR: CLOV_S5X with the row exchange of s5_row; CLOV_S5XRR: the same
-// with the CG residual update (rupd) instead of storing out (kernels of their
-// own, chosen by the launcher; not values of HopParams::clov)
-enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2, CLOV_S5X = 3, CLOV_S5XR = 4, CLOV_S5XRR = 5 };
+// with the CG residual update (rupd) instead of storing out; CLOV_XRR:
+// CLOV_X with that residual update (kernels of their own, chosen by the
+// launcher; not values of HopParams::clov)
+enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2, CLOV_S5X = 3, CLOV_S5XR = 4, CLOV_S5XRR = 5, CLOV_XRR = 6 };
 
 hipError_t launch_hop(const HopParams& P, hipStream_t st);
 hipError_t build_nbr(const Geom& g, uint4* nbr, uint32_t* nbr_neg, hipStream_t st);

@@ -45,7 +45,8 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
   if ((P.flags & HOP_RHSGROUP) && !rhsgroup_setup(Q, bs)) return hipErrorInvalidValue;
   if (P.m5post) return hipErrorInvalidValue;   // plain hops only
   if (P.s5_row && (P.clov != CLOV_S5X || P.nb != 16 || bs % 16)) return hipErrorInvalidValue;
-  if (P.rupd && (!P.s5_row || !P.R.data || !P.ralpha || !P.ractive)) return hipErrorInvalidValue;
+  if (P.rupd && (!P.R.data || !P.ralpha || !P.ractive || (P.clov == CLOV_S5X ? !P.s5_row : P.clov != CLOV_X)))
+    return hipErrorInvalidValue;
 #define CPSHIP_HOP_LAUNCH2(DAG, CLOV, SW)                                                                        \
   if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, true, CLOV, SW>), grid, block, shm, st, Q); \
   else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND_WILSON, false, CLOV, SW>), grid, block, shm, st, Q);
@@ -53,13 +54,19 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
   if (P.flags & HOP_SITEWAVE) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 1) }                                \
   else if (P.flags & HOP_RHSGROUP) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 3) }                           \
   else if (lds) {                                                                                 \
-    if constexpr (std::is_same<SIN, SpinH>::value && CLOV >= CLOV_S5X) {                              \
+    if constexpr (std::is_same<SIN, SpinH>::value && CLOV >= CLOV_S5X && CLOV <= CLOV_S5XRR) {                              \
       CPSHIP_HOP_LAUNCH2(DAG, CLOV, 2)                                                            \
     }                                                                                             \
     else return hipErrorInvalidValue;                                                             \
   } else { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 0) }
   if (P.clov == CLOV_POST) {
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, CLOV_POST) } else { CPSHIP_HOP_LAUNCH(0, CLOV_POST) }
+  } else if (P.clov == CLOV_X && P.rupd) {
+    if constexpr (std::is_same<SIN, SpinH>::value) {
+      if (P.dag) { CPSHIP_HOP_LAUNCH(1, CLOV_XRR) } else { CPSHIP_HOP_LAUNCH(0, CLOV_XRR) }
+    } else {
+      return hipErrorInvalidValue;
+    }
   } else if (P.clov == CLOV_X) {
     if (P.dag) { CPSHIP_HOP_LAUNCH(1, CLOV_X) } else { CPSHIP_HOP_LAUNCH(0, CLOV_X) }
   } else if (P.clov == CLOV_S5X && P.s5_row && P.rupd) {

@@ -1030,7 +1030,7 @@ __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem&
       // (no branch here: a conditional load splits the block and the stencil
       // above it then spills)
       R sc;
-      if constexpr (CLOV == CLOV_X) sc = R(P.scale * P.kscale[set]);
+      if constexpr (CLOV == CLOV_X || CLOV == CLOV_XRR) sc = R(P.scale * P.kscale[set]);
       else sc = R(P.scale);
       const int64_t c0 = (int64_t)(set * 2 + out_par) * CloverOf<SIN>::nplanes * P.clover.plane + site_cb;
       if constexpr (CLOV == CLOV_POST) {
@@ -1062,7 +1062,7 @@ __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem&
         if (P.m5post) m5_row<DAG>(P, j, res);
       }
     }
-    if constexpr (CLOV == CLOV_S5XRR) {
+    if constexpr (CLOV == CLOV_S5XRR || CLOV == CLOV_XRR) {
       // the CG step r -= alpha (M^dag M p) and |r|^2, with M^dag M p in FP32
       if (P.ractive[rhs]) {
         const R a = R(P.ralpha[rhs]);

@@ -188,6 +188,11 @@ struct HopClover {
   const double* kscale;
   int rhs_per_set;
   int rhs0;   // global index of local rhs 0 (the CG's active window)
+  // optional (CLOV_X, FP16): instead of storing out, the CG residual step
+  // r -= ralpha[rhs] * out for the rhs with ractive[rhs], partials |r|^2
+  const FieldRef* R = nullptr;
+  const double* ralpha = nullptr;
+  const int* ractive = nullptr;
 };
 int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
            double scale, const FieldRef& X, double xcoef, double* partial, int nrhs, const HopClover* hc);
@@ -214,6 +219,8 @@ int clover_matpc(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& ou
                  const FieldRef& t1, int dag, double* partial, int nrhs);
 int clover_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
                  const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs);
+int clover_mdag_rupdate(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& r, const FieldRef& t2,
+                        const FieldRef& t1, const double* alpha, const int* active, double* partial, int nrhs);
 int clover_build(Lattice& L, const double* kappas, int n_kappa, double csw);
 int clover_export(Lattice& L, int k, int cb, double* A);
 // MatInv source preparation (src = MatPcDag of the prepared odd source) and

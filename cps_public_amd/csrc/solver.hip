@@ -285,6 +285,12 @@ int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, con
     P.kscale = hc->kscale;
     P.rhs_per_set = hc->rhs_per_set;
     P.rhs0 = hc->rhs0;
+    if (hc->R) {
+      P.rupd = 1;
+      P.R = *hc->R;
+      P.ralpha = hc->ralpha;
+      P.ractive = hc->ractive;
+    }
   }
   rc = tuned_launch(L, P);
   return rc;
@@ -983,6 +989,16 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     const char* e = getenv("CPS_HIP_R_IN_HOP");
     return s5_in_xpay && svec == STORE_H && !(e && *e == '0');
   }();
+  // clover, FP16 directions and residual, on request
+  // (CPS_HIP_CLOVER_R_IN_HOP=1): likewise the residual step in the epilogue
+  // of the last hop of MatPcDag, after the alpha of the first half.  Off by
+  // default: that epilogue (clover multiply, X and r in registers) spills
+  // at the hop's register budget and the C4 solve takes 3.85 instead of
+  // 2.56 s.
+  const bool clover_rih = [&] {
+    const char* e = getenv("CPS_HIP_CLOVER_R_IN_HOP");
+    return lagged && op->kind == CPS_HIP_OP_CLOVER && sps == STORE_H && svec == STORE_H && e && *e == '1';
+  }();
   Field& t3s = L.field_min(S_T3S, sps, n);
   if (L.last_rc) return L.last_rc;
   // FP16 directions and an FP32 solution: iterations run in pairs with the
@@ -1036,6 +1052,16 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
     // the operator on this iteration's direction pw, alpha, the residual step
     // (in the last hop of M^dag, or its own pass) and beta
     auto opstep = [&](const FieldRef& pw) -> int {
+      if (clover_rih) {
+        // M p with |M p|^2, alpha, then r -= alpha M^dag (M p) with |r|^2
+        int e = clover_matpc(L, op, hprec, t3s.ref(), pw, t1s.ref(), 0, part, nw);
+        if (e) return e;
+        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
+        e = clover_mdag_rupdate(L, op, hprec, rsw, t3s.ref(), t1s.ref(), sw.alpha, sw.active, part, nw);
+        if (e) return e;
+        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_BETA, sw, L.stream));
+        return 0;
+      }
       if (r_in_hop) {
         // M p with |M p|^2, alpha, then r -= alpha M^dag (M p) with |r|^2
         int e = mobius_m(L, op, hprec, t3s.ref(), pw, t1s.ref(), t2s.ref(), 0, part, nw, true);
@@ -1084,6 +1110,7 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
                                        bsb, L.stream));
       else
         CPSHIP_CHECK(blas_xpay_mixed(sps, psw, svec, rsw, sw.beta, sw.active, o.per_rhs, nw, bsb, L.stream));
+      if (clover_rih) return opstep(psw);
       if (r_in_hop) {
         // M p with |M p|^2, alpha, then r -= alpha M^dag (M p) with |r|^2
         int e = mobius_m(L, op, hprec, t3s.ref(), psw, t1s.ref(), t2s.ref(), 0, part, nw, true);

@@ -947,6 +947,39 @@ def test_rhsgroup_mapping_bit_identical(monkeypatch, case):
         assert np.array_equal(a, b_)
 
 
+@pytest.mark.parametrize("paired", ["1", "0"])
+def test_clover_mixed_residual_step_in_hop(monkeypatch, paired):
+    """Clover with two kappa sets, half precision: r -= alpha M^dag M p and
+    |r|^2 in the epilogue of MatPcDag's last hop (CLOV_X with the residual
+    update, CPS_HIP_CLOVER_R_IN_HOP=1; M^dag M p never stored), with paired
+    and with single solution steps.  It rounds M^dag M p differently from the separate pass, so the
+    checks are the FP64 stopping rule, iteration counts within 5% and the
+    solutions within what the 1e-10 residual bound allows."""
+    monkeypatch.setenv("CPS_HIP_PAIRED_X", paired)
+    L = [4, 4, 4, 8]
+    u = wilson_gauge(L, 0.2, 270)
+    nrhs = 4
+    vcb = int(np.prod(L)) // 2
+    src = np.concatenate([F.gaussian_fermion(vcb, 271 + k) * (1.0 + 3.0 * k) for k in range(nrhs)])
+    arg = C.make_cg_arg(1e-10, 2000, precision=C.PREC_MIXED_HALF, reliable_every=40)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("CPS_HIP_CLOVER_R_IN_HOP", fused)
+        lat = C.Lattice(L, 1, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        lat.clover_build(CLOVER_KAPPAS, CLOVER_CSW)
+        out[fused] = lat.invcg(clover_op(CLOVER_KAPPAS[0]), src, arg, nrhs=nrhs)
+        lat.close()
+    it1 = [r.iterations for r in out["1"][1]]
+    it0 = [r.iterations for r in out["0"][1]]
+    print("clover r in hop", it1, it0, flush=True)
+    assert max(r.true_res for r in out["1"][1]) <= 1e-10
+    assert all(r.status == 1 for r in out["1"][1])
+    for a, b_ in zip(it1, it0):
+        assert abs(a - b_) <= 0.05 * b_ + 2
+    assert rel(out["1"][0], out["0"][0]) < 1e-8
+
+
 @pytest.mark.parametrize("case", ["clover", "wilson"])
 def test_mixed_cg_mapping_consistent(monkeypatch, case):
     """A mixed-precision CG with the 4-D operators' hops free to use the
