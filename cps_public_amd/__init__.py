@@ -82,6 +82,8 @@ def _declare(l):
         "cps_hip_lattice_create": [P(vp), vp, i, vp],
         "cps_hip_lattice_destroy": [vp],
         "cps_hip_gauge_import": [vp, vp],
+        "cps_hip_lattice_set_twist": [vp, vp],
+        "cps_hip_gauge_boundary_phase": [vp, vp, P(i)],
         "cps_hip_wilson_dslash": [vp, vp, vp, vp, i, i],
         "cps_hip_wilson_m": [vp, vp, vp, vp, d],
         "cps_hip_wilson_mdag": [vp, vp, vp, vp, d],
@@ -170,7 +172,10 @@ class Lattice:
     """Device handle: geometry, resident gauge field and scratch (wilson_init /
     dwf_init equivalent).  node_sites = (Lx, Ly, Lz, Lt)."""
 
-    def __init__(self, node_sites, ls=1, bc=(0, 0, 0, 1)):
+    def __init__(self, node_sites, ls=1, bc=(0, 0, 0, 1), twist=None):
+        """bc[mu]: CPS BndCndType, 0 periodic, 1 antiperiodic, 2 twisted
+        (twist[mu] = GJP.TwistAngle in units of pi); used by the CANONICAL
+        import only."""
         self.L = [int(v) for v in node_sites]
         self.ls = int(ls)
         self.bc = [int(v) for v in bc]
@@ -179,6 +184,17 @@ class Lattice:
         bcs = (ctypes.c_int * 4)(*self.bc)
         _check(lib().cps_hip_lattice_create(ctypes.byref(h), dims, self.ls, bcs), "cps_hip_lattice_create")
         self.h = h
+        if twist is not None:
+            tw = np.ascontiguousarray(np.asarray(twist, dtype=np.float64))
+            _check(lib().cps_hip_lattice_set_twist(self.h, _ptr(tw)), "cps_hip_lattice_set_twist")
+
+    def boundary_phase(self):
+        """det U of the last-slice links per direction (complex array of 4)
+        and whether the field is SU(3) times those factors (recon_ok)."""
+        ph = np.zeros(8)
+        ok = ctypes.c_int()
+        _check(lib().cps_hip_gauge_boundary_phase(self.h, _ptr(ph), ctypes.byref(ok)), "cps_hip_gauge_boundary_phase")
+        return ph[0::2] + 1j * ph[1::2], bool(ok.value)
 
     @property
     def vcb(self):
@@ -208,10 +224,10 @@ class Lattice:
                "cps_hip_wilson_dslash")
         return chi
 
-    def wilson_m(self, psi, kappa, dag=0):
+    def wilson_m(self, psi, kappa, dag=0, u=None):
         chi = np.zeros(24 * self.vcb)
         f = lib().cps_hip_wilson_mdag if dag else lib().cps_hip_wilson_m
-        _check(f(self.h, _ptr(chi), None, _ptr(psi), kappa), "cps_hip_wilson_m")
+        _check(f(self.h, _ptr(chi), None if u is None else _ptr(u), _ptr(psi), kappa), "cps_hip_wilson_m")
         return chi
 
     def wilson_mdagm(self, psi, kappa):

@@ -19,8 +19,9 @@ namespace {
 
 // One thread per (canonical site, mu): U_mu(x) to the WILSON array
 // [parity][site_cb][mu][col][row][re,im] (the transpose of convert_func.C:
-// 402-404), times -1 on the last slice of an antiperiodic direction
-// (Lattice::BondCond, lattice_base.C:5313-5369).
+// 402-404), times the boundary factor g.bf[mu] on the last slice of the
+// direction: -1 antiperiodic, the twist phase e^{-i pi theta} twisted
+// (Lattice::BondCond, lattice_base.C:5313-5369; GJP::TwistPhase, gjp.C:475).
 __global__ void k_gauge_canon_to_wilson(const double* __restrict__ canon, double* __restrict__ wil, Geom g) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t V = 2 * (int64_t)g.Vcb;
@@ -36,15 +37,17 @@ __global__ void k_gauge_canon_to_wilson(const double* __restrict__ canon, double
   }
   const int p = (c[0] + c[1] + c[2] + c[3]) & 1;
   const int site_cb = (c[0] >> 1) + (g.L[0] >> 1) * (c[1] + g.L[1] * (c[2] + g.L[2] * c[3]));
-  const double sign = (g.bc[mu] && c[mu] == g.L[mu] - 1) ? -1.0 : 1.0;
+  const bool last = c[mu] == g.L[mu] - 1;
+  const double fr = last ? g.bf[mu][0] : 1.0, fi = last ? g.bf[mu][1] : 0.0;
   const double* src = canon + 72 * site + 18 * mu;
   double* dst = wil + 72 * ((int64_t)p * g.Vcb + site_cb) + 18 * mu;
 #pragma unroll
   for (int row = 0; row < 3; row++)
 #pragma unroll
     for (int col = 0; col < 3; col++) {
-      dst[2 * (row + 3 * col)] = sign * src[2 * (3 * row + col)];
-      dst[2 * (row + 3 * col) + 1] = sign * src[2 * (3 * row + col) + 1];
+      const double a = src[2 * (3 * row + col)], b = src[2 * (3 * row + col) + 1];
+      dst[2 * (row + 3 * col)] = fr * a - fi * b;
+      dst[2 * (row + 3 * col) + 1] = fr * b + fi * a;
     }
 }
 

@@ -114,6 +114,22 @@ int cps_hip_lattice_destroy(cps_hip_lattice* lat) {
   return 0;
 }
 
+int cps_hip_lattice_set_twist(cps_hip_lattice* lat, const double twist_angle[4]) {
+  if (check_lat(lat)) return CPS_HIP_ERR_ARG;
+  return lat->L.set_twist(twist_angle);
+}
+
+int cps_hip_gauge_boundary_phase(const cps_hip_lattice* lat, double phase[8], int* recon_ok) {
+  if (!lat || !phase) return CPS_HIP_ERR_ARG;
+  if (!lat->L.has_gauge) return CPS_HIP_ERR_NOGAUGE;
+  for (int mu = 0; mu < 4; mu++) {
+    phase[2 * mu] = lat->L.link_phase[mu][0];
+    phase[2 * mu + 1] = lat->L.link_phase[mu][1];
+  }
+  if (recon_ok) *recon_ok = lat->L.recon_ok ? 1 : 0;
+  return 0;
+}
+
 int cps_hip_gauge_import(cps_hip_lattice* lat, const double* u) {
   if (check_lat(lat) || !u) return CPS_HIP_ERR_ARG;
   return lat->L.import_gauge(u);
@@ -271,7 +287,7 @@ int cps_hip_matpc_dag_matpc_prec(cps_hip_lattice* lat, const cps_hip_op* op, int
   if (!L.has_gauge) return CPS_HIP_ERR_NOGAUGE;
   if (precision < CPS_HIP_PREC_DOUBLE || precision > CPS_HIP_PREC_MIXED_SINGLE) return CPS_HIP_ERR_ARG;
   int hprec, st;
-  precision_formats(precision, op, L.ls, nrhs, &st, &hprec);
+  precision_formats(precision, op, L.ls, nrhs, L.recon_ok, &st, &hprec);
   OpLayout o;
   int rc = op_layout(op, L.ls, &o, L.g);
   if (rc) return rc;
@@ -470,7 +486,7 @@ int cps_hip_bench_hop(cps_hip_lattice* lat, int kind, int precision, int nrhs, i
   if (rc) return rc;
   if (precision < CPS_HIP_PREC_DOUBLE || precision > CPS_HIP_PREC_MIXED_SINGLE) return CPS_HIP_ERR_ARG;
   int hprec, st;
-  precision_formats(precision, &op, L.ls, nrhs, &st, &hprec);
+  precision_formats(precision, &op, L.ls, nrhs, L.recon_ok, &st, &hprec);
   const int64_t n = o.per_rhs * nrhs;
   Field& fi = L.field(S_BENCH_IN, st, n);
   Field& fo = L.field(S_BENCH_OUT, st, n);

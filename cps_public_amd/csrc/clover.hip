@@ -51,7 +51,7 @@ __device__ __forceinline__ void link_at(const GaugeRef& u, const Geom& g, const 
     y[d] = v;
   }
   const int par = (y[0] + y[1] + y[2] + y[3]) & 1;
-  link_load<LinkD18, double>(u, par, mu, cb_index(g, y), g.Vcb, 1.0, m);
+  link_load<LinkD18, double>(u, par, mu, cb_index(g, y), g.Vcb, make_float2(1.f, 0.f), m);
 }
 
 // 2x2 spin block (chirality beta: 0 spins 0,1; 1 spins 2,3) of

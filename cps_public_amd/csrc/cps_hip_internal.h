@@ -43,7 +43,11 @@ namespace cpship {
 struct Geom {
   int L[4];     // local lattice extent x, y, z, t
   int Vcb;      // V/2
-  int bc[4];    // 1 = antiperiodic in direction mu (sign folded into links)
+  int bc[4];    // boundary condition of direction mu (CPS BndCndType: 0
+                // periodic, 1 antiperiodic, 2 twisted), applied to the links
+                // by the CANONICAL gauge import
+  double bf[4][2];   // factor (re, im) of the links of the last mu slice that
+                     // the CANONICAL import applies (1, -1 or the twist phase)
 };
 
 // ---- scalar / complex helpers ------------------------------------------
@@ -263,11 +267,29 @@ struct GaugeRef {
   int64_t plane;    // = 8 * Vcb links per plane
 };
 
+// Third row of a link stored by its first two rows: for any unitary U,
+// row2 = det(U) conj(row0 x row1).  ph is det(U): 1 for SU(3) links, the
+// boundary factor (-1 antiperiodic, e^{3 i theta} twisted by e^{i theta})
+// for the links of the last slice (HopParams::bph).
+template <typename R>
+__device__ __forceinline__ void link_row2(cplx<R> u[9], float2 ph) {
+  const R pr = R(ph.x), pi = R(ph.y);
+  const R a0 = u[1].re * u[5].re - u[1].im * u[5].im - u[2].re * u[4].re + u[2].im * u[4].im;
+  const R b0 = -(u[1].re * u[5].im + u[1].im * u[5].re - u[2].re * u[4].im - u[2].im * u[4].re);
+  const R a1 = u[2].re * u[3].re - u[2].im * u[3].im - u[0].re * u[5].re + u[0].im * u[5].im;
+  const R b1 = -(u[2].re * u[3].im + u[2].im * u[3].re - u[0].re * u[5].im - u[0].im * u[5].re);
+  const R a2 = u[0].re * u[4].re - u[0].im * u[4].im - u[1].re * u[3].re + u[1].im * u[3].im;
+  const R b2 = -(u[0].re * u[4].im + u[0].im * u[4].re - u[1].re * u[3].im - u[1].im * u[3].re);
+  u[6] = cmk<R>(pr * a0 - pi * b0, pr * b0 + pi * a0);
+  u[7] = cmk<R>(pr * a1 - pi * b1, pr * b1 + pi * a1);
+  u[8] = cmk<R>(pr * a2 - pi * b2, pr * b2 + pi * a2);
+}
+
 // Load link (parity, mu, site_cb) as a full 3x3 matrix u[row*3+col].
-// sign: boundary sign for the reconstructed third row of a 12-real link.
+// ph: factor of the reconstructed third row of a 12-real link (link_row2).
 template <typename LK, typename R>
 __device__ __forceinline__ void link_load(const GaugeRef& g, int parity, int mu, int site, int Vcb,
-                                          R sign, cplx<R> u[9]) {
+                                          float2 ph, cplx<R> u[9]) {
   const int64_t idx = (int64_t)(parity * 4 + mu) * Vcb + site;
   if constexpr (std::is_same<LK, LinkD18>::value) {
     const double2* p = reinterpret_cast<const double2*>(g.data) + idx;
@@ -296,13 +318,7 @@ __device__ __forceinline__ void link_load(const GaugeRef& g, int parity, int mu,
         u[k] = cmk<R>(R(h.x), R(h.y));
       }
     }
-    // row2 = sign * conj(row0 x row1)
-    u[6] = cmk<R>(sign * (u[1].re * u[5].re - u[1].im * u[5].im - u[2].re * u[4].re + u[2].im * u[4].im),
-                  -sign * (u[1].re * u[5].im + u[1].im * u[5].re - u[2].re * u[4].im - u[2].im * u[4].re));
-    u[7] = cmk<R>(sign * (u[2].re * u[3].re - u[2].im * u[3].im - u[0].re * u[5].re + u[0].im * u[5].im),
-                  -sign * (u[2].re * u[3].im + u[2].im * u[3].re - u[0].re * u[5].im - u[0].im * u[5].re));
-    u[8] = cmk<R>(sign * (u[0].re * u[4].re - u[0].im * u[4].im - u[1].re * u[3].re + u[1].im * u[3].im),
-                  -sign * (u[0].re * u[4].im + u[0].im * u[4].re - u[1].re * u[3].im - u[1].im * u[3].re));
+    link_row2<R>(u, ph);
   }
 }
 
@@ -398,11 +414,11 @@ __device__ __forceinline__ void link_raw_load(LinkRaw<LK>& r, const GaugeRef& g,
 }
 
 template <typename LK, typename R>
-__device__ __forceinline__ void link_decode(const LinkRaw<LK>& r, R sign, cplx<R> u[9]) {
+__device__ __forceinline__ void link_decode(const LinkRaw<LK>& r, float2 ph, cplx<R> u[9]) {
   if constexpr (std::is_same<LK, LinkD18>::value) {
 #pragma unroll
     for (int k = 0; k < 9; k++) u[k] = cmk<R>(R(r.q[k].x), R(r.q[k].y));
-    (void)sign;
+    (void)ph;
   } else if constexpr (LinkIs18Half<LK>::value) {
     const uint32_t w[9] = {r.a.x, r.a.y, r.a.z, r.a.w, r.b.x, r.b.y, r.b.z, r.b.w, r.c};
 #pragma unroll
@@ -410,7 +426,7 @@ __device__ __forceinline__ void link_decode(const LinkRaw<LK>& r, R sign, cplx<R
       const float2 h = LinkIsBf16<LK>::value ? bf2f(w[k]) : h2f(w[k]);
       u[k] = cmk<R>(R(h.x), R(h.y));
     }
-    (void)sign;
+    (void)ph;
   } else {
     if constexpr (std::is_same<LK, LinkF12>::value) {
 #pragma unroll
@@ -426,12 +442,7 @@ __device__ __forceinline__ void link_decode(const LinkRaw<LK>& r, R sign, cplx<R
         u[k] = cmk<R>(R(h.x), R(h.y));
       }
     }
-    u[6] = cmk<R>(sign * (u[1].re * u[5].re - u[1].im * u[5].im - u[2].re * u[4].re + u[2].im * u[4].im),
-                  -sign * (u[1].re * u[5].im + u[1].im * u[5].re - u[2].re * u[4].im - u[2].im * u[4].re));
-    u[7] = cmk<R>(sign * (u[2].re * u[3].re - u[2].im * u[3].im - u[0].re * u[5].re + u[0].im * u[5].im),
-                  -sign * (u[2].re * u[3].im + u[2].im * u[3].re - u[0].re * u[5].im - u[0].im * u[5].re));
-    u[8] = cmk<R>(sign * (u[0].re * u[4].re - u[0].im * u[4].im - u[1].re * u[3].re + u[1].im * u[3].im),
-                  -sign * (u[0].re * u[4].im + u[0].im * u[4].re - u[1].re * u[3].im - u[1].im * u[3].re));
+    link_row2<R>(u, ph);
   }
 }
 

@@ -50,7 +50,11 @@ struct HopParams {
   int out_cb5;       // DWF: 5-D checkerboard of the output field
   int ls;
   int block;         // threads per block (divides nsites_out*nb)
-  int link_bc_sign;  // 12-real links: apply the antiperiodic sign on reconstruction
+  // 12-real links: factor (re, im) of the rebuilt third row of a link that
+  // crosses the lattice boundary in direction mu (the det of such links, set
+  // from the gauge data at import: -1 antiperiodic, e^{3 i theta} for a link
+  // twisted by e^{i theta}); interior links use 1
+  float bph[4][2];
   double two_over_a5, neg_mass_two_over_a5;
   double scale;      // out = xcoef*X + scale*hop
   double xcoef;

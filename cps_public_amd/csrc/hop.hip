@@ -74,8 +74,9 @@ hipError_t launch_hop(const HopParams& P, hipStream_t st) {
 
 // Neighbour table: for every (parity p, site_cb s) at ns = p*Vcb + s, the
 // checkerboard indices of the 8 neighbours (d = 0..3: +mu, 4..7: -mu) in
-// nbr[ns] (d 0-3) and nbr[2 Vcb + ns] (d 4-7), and the directions whose link
-// crosses an anti-periodic boundary as bits of nbr_neg[ns].
+// nbr[ns] (d 0-3) and nbr[2 Vcb + ns] (d 4-7), and as bits of nbr_neg[ns]
+// the directions whose link lies on the last slice of its direction (the
+// links that carry the boundary factor; Nbr::neg).
 __global__ void k_build_nbr(Geom g, uint4* __restrict__ nbr, uint32_t* __restrict__ nbr_neg) {
   const int64_t ns = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (ns >= 2 * (int64_t)g.Vcb) return;

@@ -122,7 +122,8 @@ __device__ __forceinline__ int cb_index(const Geom& g, const int x[4]) {
 // is -mu (mu = d-4), the CPS summation order (wilson_dslash.C:437-447).
 struct Nbr {
   int idx[8];      // checkerboard index of the neighbour site
-  unsigned neg;    // bit d: the link of direction d carries the boundary sign
+  unsigned neg;    // bit d: the link of direction d lies on the last slice of
+                   // direction d & 3 (it carries the boundary factor)
 };
 
 __device__ __forceinline__ Nbr neighbours(const Geom& g, const Coord& c) {
@@ -134,14 +135,21 @@ __device__ __forceinline__ Nbr neighbours(const Geom& g, const Coord& c) {
     int n[4] = {c.x[0], c.x[1], c.x[2], c.x[3]};
     if (d < 4) {
       n[mu] = (n[mu] + 1 == g.L[mu]) ? 0 : n[mu] + 1;
-      if (g.bc[mu] && c.x[mu] == g.L[mu] - 1) N.neg |= 1u << d;
+      if (c.x[mu] == g.L[mu] - 1) N.neg |= 1u << d;
     } else {
       n[mu] = (n[mu] == 0) ? g.L[mu] - 1 : n[mu] - 1;
-      if (g.bc[mu] && c.x[mu] == 0) N.neg |= 1u << d;
+      if (c.x[mu] == 0) N.neg |= 1u << d;
     }
     N.idx[d] = cb_index(g, n);
   }
   return N;
+}
+
+// Third-row factor of the link of direction d (bit d of the site's boundary
+// mask: the link lies on the last slice of direction d & 3)
+__device__ __forceinline__ float2 link_phase(const HopParams& P, unsigned bmask, int d) {
+  const int mu = d & 3;
+  return ((bmask >> d) & 1u) ? make_float2(P.bph[mu][0], P.bph[mu][1]) : make_float2(1.f, 0.f);
 }
 
 // ---- packed half-precision path ----------------------------------------------
@@ -207,29 +215,30 @@ __device__ __forceinline__ void reconstruct_scaled(cplx<float> acc[12], const cp
   }
 }
 
-// third row of a 12-real link, sign * conj(row0 x row1), from f32 values
-__device__ __forceinline__ void row2_f32(const float2 u[6], float sign, float2 r[3]) {
-  r[0] = make_float2(sign * (u[1].x * u[5].x - u[1].y * u[5].y - u[2].x * u[4].x + u[2].y * u[4].y),
-                     -sign * (u[1].x * u[5].y + u[1].y * u[5].x - u[2].x * u[4].y - u[2].y * u[4].x));
-  r[1] = make_float2(sign * (u[2].x * u[3].x - u[2].y * u[3].y - u[0].x * u[5].x + u[0].y * u[5].y),
-                     -sign * (u[2].x * u[3].y + u[2].y * u[3].x - u[0].x * u[5].y - u[0].y * u[5].x));
-  r[2] = make_float2(sign * (u[0].x * u[4].x - u[0].y * u[4].y - u[1].x * u[3].x + u[1].y * u[3].y),
-                     -sign * (u[0].x * u[4].y + u[0].y * u[4].x - u[1].x * u[3].y - u[1].y * u[3].x));
+// third row of a 12-real link, ph * conj(row0 x row1), from f32 values (ph
+// = det U, see link_row2)
+__device__ __forceinline__ void row2_f32(const float2 u[6], float2 ph, float2 r[3]) {
+  cplx<float> m[9];
+#pragma unroll
+  for (int k = 0; k < 6; k++) m[k] = cmk<float>(u[k].x, u[k].y);
+  link_row2<float>(m, ph);
+#pragma unroll
+  for (int k = 0; k < 3; k++) r[k] = make_float2(m[6 + k].re, m[6 + k].im);
 }
 
 // link elements as packed 32-bit words (f16 or bf16 pairs), third row rebuilt for 12-real storage
 template <typename LK>
-__device__ __forceinline__ void link_words(const LinkRaw<LK>& r, float sign, uint32_t U[9]) {
+__device__ __forceinline__ void link_words(const LinkRaw<LK>& r, float2 ph, uint32_t U[9]) {
   if constexpr (LinkIs18Half<LK>::value) {
     U[0] = r.a.x; U[1] = r.a.y; U[2] = r.a.z; U[3] = r.a.w;
     U[4] = r.b.x; U[5] = r.b.y; U[6] = r.b.z; U[7] = r.b.w; U[8] = r.c;
-    (void)sign;
+    (void)ph;
   } else {
     U[0] = r.a.x; U[1] = r.a.y; U[2] = r.a.z; U[3] = r.a.w; U[4] = r.b.x; U[5] = r.b.y;
     float2 f[6], r2[3];
 #pragma unroll
     for (int k = 0; k < 6; k++) f[k] = LinkIsBf16<LK>::value ? bf2f(U[k]) : h2f(U[k]);
-    row2_f32(f, sign, r2);
+    row2_f32(f, ph, r2);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       if constexpr (LinkIsBf16<LK>::value) {
@@ -298,8 +307,8 @@ __device__ __forceinline__ float dot2_first(uint32_t a, uint32_t b) {
 
 // link words as f16 pairs (bf16 storage is widened to f16 here)
 template <typename LK>
-__device__ __forceinline__ void link_words_f16(const LinkRaw<LK>& r, float sign, uint32_t U[9]) {
-  link_words<LK>(r, sign, U);
+__device__ __forceinline__ void link_words_f16(const LinkRaw<LK>& r, float2 ph, uint32_t U[9]) {
+  link_words<LK>(r, ph, U);
   if constexpr (LinkIsBf16<LK>::value) {
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -350,92 +359,26 @@ __device__ __forceinline__ void dir_compute_h2w(const SpinRaw<SpinH>& sr, const 
 }
 
 template <int D, typename LK, int DAG>
-__device__ __forceinline__ void dir_compute_h2(const SpinRaw<SpinH>& sr, const LinkRaw<LK>& lr, float sign,
+__device__ __forceinline__ void dir_compute_h2(const SpinRaw<SpinH>& sr, const LinkRaw<LK>& lr, float2 ph,
                                                cplx<float> acc[12]) {
   uint32_t U[9];
-  link_words_f16<LK>(lr, sign, U);
+  link_words_f16<LK>(lr, ph, U);
   dir_compute_h2w<D, DAG>(sr, U, acc);
-}
-
-template <int D, typename LK, int DAG>
-__device__ __forceinline__ void dir_compute_h(const SpinRaw<SpinH>& sr, const LinkRaw<LK>& lr, float sign,
-                                              cplx<float> acc[12]) {
-  constexpr int MU = D & 3;
-  constexpr int S = (DAG == 0) ? 1 : -1;
-  constexpr int A = (D < 4) ? -S : S;
-  hv2 h[6];
-  project_h<MU, A>(sr, h);
-  uint32_t U[9];
-  link_words<LK>(lr, sign, U);
-  cplx<float> w[6];
-  if constexpr (LinkIsBf16<LK>::value) {
-    bv2 hb[6], hx[6];
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-      const float2 f = make_float2((float)h[k].x, (float)h[k].y);
-      if constexpr (D < 4) {   // U h: re = (Ur,Ui).(hr,-hi), im = (Ur,Ui).(hi,hr)
-        hb[k] = (bv2){(__bf16)f.x, (__bf16)(-f.y)};
-        hx[k] = (bv2){(__bf16)f.y, (__bf16)f.x};
-      } else {                 // U^+ h: re = (Ur,Ui).(hr,hi), im = (Ur,Ui).(hi,-hr)
-        hb[k] = (bv2){(__bf16)f.x, (__bf16)f.y};
-        hx[k] = (bv2){(__bf16)f.y, (__bf16)(-f.x)};
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-      for (int r = 0; r < 3; r++) {
-        float re = 0.f, im = 0.f;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          const bv2 u = as_bv2(D < 4 ? U[3 * r + c] : U[3 * c + r]);
-          re = __builtin_amdgcn_fdot2_f32_bf16(u, hb[3 * s + c], re, false);
-          im = __builtin_amdgcn_fdot2_f32_bf16(u, hx[3 * s + c], im, false);
-        }
-        w[3 * s + r] = cmk<float>(re, im);
-      }
-  } else {
-    hv2 hb[6], hx[6];
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-      if constexpr (D < 4) {
-        hb[k] = h[k] * (hv2){(_Float16)1, (_Float16)(-1)};
-        hx[k] = hswap(h[k]);
-      } else {
-        hb[k] = h[k];
-        hx[k] = hswap(h[k]) * (hv2){(_Float16)1, (_Float16)(-1)};
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-      for (int r = 0; r < 3; r++) {
-        float re = 0.f, im = 0.f;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          const hv2 u = as_hv2(D < 4 ? U[3 * r + c] : U[3 * c + r]);
-          re = __builtin_amdgcn_fdot2(u, hb[3 * s + c], re, false);
-          im = __builtin_amdgcn_fdot2(u, hx[3 * s + c], im, false);
-        }
-        w[3 * s + r] = cmk<float>(re, im);
-      }
-  }
-  reconstruct_scaled<MU, A>(acc, w, sr.n);
 }
 
 // forward:  U_mu(x) (1 - s gamma_mu) psi(x+mu)        (link of the output site)
 // backward: U_mu(x-mu)^dagger (1 + s gamma_mu) psi(x-mu) (link of the neighbour)
 template <int D, typename SIN, typename LK, typename R, int DAG>
-__device__ __forceinline__ void dir_compute(const SpinRaw<SIN>& sr, const LinkRaw<LK>& lr, R sign, cplx<R> acc[12]) {
+__device__ __forceinline__ void dir_compute(const SpinRaw<SIN>& sr, const LinkRaw<LK>& lr, float2 ph, cplx<R> acc[12]) {
   if constexpr (std::is_same<SIN, SpinH>::value) {
-    dir_compute_h2<D, LK, DAG>(sr, lr, float(sign), acc);
+    dir_compute_h2<D, LK, DAG>(sr, lr, ph, acc);
     return;
   }
   constexpr int MU = D & 3;
   constexpr int S = (DAG == 0) ? 1 : -1;
   cplx<R> psi[12], h[6], w[6], u[9];
   raw_decode<SIN, R>(sr, psi);
-  link_decode<LK, R>(lr, sign, u);
+  link_decode<LK, R>(lr, ph, u);
   if constexpr (D < 4) {
     project<MU, -S, R>(psi, h);
     su3_mul<R>(u, h, w);
@@ -466,8 +409,8 @@ __device__ __forceinline__ void hop_batch(const HopParams& P, const Nbr& N, int 
   }
 #define CPSHIP_DIR(K)                                                                             \
   if constexpr ((K) < NB) {                                                                       \
-    const R sg = (P.link_bc_sign && ((N.neg >> (D0 + (K))) & 1u)) ? R(-1) : R(1);                 \
-    dir_compute<D0 + (K), SIN, LK, R, DAG>(sr[K], lr[K], sg, acc);                                \
+    const float2 ph = link_phase(P, N.neg, D0 + (K));                                             \
+    dir_compute<D0 + (K), SIN, LK, R, DAG>(sr[K], lr[K], ph, acc);                                \
   }
   CPSHIP_DIR(0) CPSHIP_DIR(1) CPSHIP_DIR(2) CPSHIP_DIR(3) CPSHIP_DIR(4) CPSHIP_DIR(5) CPSHIP_DIR(6) CPSHIP_DIR(7)
 #undef CPSHIP_DIR
@@ -714,7 +657,7 @@ __device__ __forceinline__ HopElem hop_decode(const HopParams& P, int64_t sb64, 
 // LDS staging of the links (HOP_LDS_LINKS): thread t < 8 nsl loads the link
 // of direction d = t % 8 of block site sl = t / 8, as hop_batch would, and
 // stores it in the f16 word form of the dot2 product (third row rebuilt,
-// boundary sign applied).  Every thread of the block calls it; the caller
+// boundary factor applied).  Every thread of the block calls it; the caller
 // synchronises the block before reading.
 template <typename LK, int KIND>
 __device__ __forceinline__ void lds_fill_links(const HopParams& P, const HopElem& E, uint32_t* lds) {
@@ -740,11 +683,11 @@ __device__ __forceinline__ void lds_fill_links(const HopParams& P, const HopElem
     const uint32_t nidx = reinterpret_cast<const uint32_t*>(P.nbr)[(2 * (int64_t)g.Vcb + ns) * 4 + mu];
     lidx = (int64_t)((1 - out_par) * 4 + mu) * g.Vcb + nidx;
   }
-  const float sign = (P.link_bc_sign && ((P.nbr_neg[ns] >> d) & 1u)) ? -1.f : 1.f;
+  const float2 ph = link_phase(P, P.nbr_neg[ns], d);
   LinkRaw<LK> lr;
   link_raw_load<LK>(lr, P.gauge, lidx);
   uint32_t U[9];
-  link_words_f16<LK>(lr, sign, U);
+  link_words_f16<LK>(lr, ph, U);
   uint4* w = reinterpret_cast<uint4*>(lds + sl * LDS_SITE_WORDS + 8 * d);
   w[0] = make_uint4(U[0], U[1], U[2], U[3]);
   w[1] = make_uint4(U[4], U[5], U[6], U[7]);

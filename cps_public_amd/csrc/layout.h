@@ -14,6 +14,14 @@ hipError_t unpack_spinor(int storage, const FieldRef& in, double* aos, int map, 
                          int64_t n, hipStream_t st);
 hipError_t convert_spinor(int sin, const FieldRef& in, int sout, const FieldRef& out, int64_t n, hipStream_t st);
 hipError_t pack_gauge(int link_storage, const double* aos, const GaugeRef& out, int Vcb, hipStream_t st);
+// Determinants of the WILSON-order FP64 links aos: ref[mu] (2 doubles each)
+// receives det U of the mu-link at x_mu = L_mu - 1, other coordinates 0;
+// dev[2 mu + b] (as float bits, device memory zeroed by the caller) the
+// largest |det U - e| over interior (b = 0, e = 1) and last-slice (b = 1,
+// e = ref[mu]) links of direction mu.
+// *flag (device) = 1 if the n 64-bit words of a and b differ anywhere, else 0
+hipError_t words_differ(const uint64_t* a, const uint64_t* b, int64_t n, unsigned* flag, hipStream_t st);
+hipError_t link_det_check(const double* aos, const Geom& g, double* ref, unsigned* dev, hipStream_t st);
 
 inline size_t spin_elem_bytes(int storage) {
   return storage == STORE_D ? 16 : storage == STORE_F ? 8 : 4;

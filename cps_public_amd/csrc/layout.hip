@@ -11,6 +11,8 @@
 #include "cps_hip_internal.h"
 #include "layout.h"
 
+#include <algorithm>
+
 namespace cpship {
 
 __device__ __forceinline__ int64_t soa_index(int map, int64_t k, int Vcb, int ls, int cb5, int nrhs_unused) {
@@ -186,6 +188,104 @@ hipError_t pack_gauge(int link_storage, const double* aos, const GaugeRef& out, 
     case LINK_B18: hipLaunchKernelGGL(k_pack_gauge<LinkB18>, dim3(nblocks(n, bs)), dim3(bs), 0, st, aos, out, Vcb); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+// det of the 3x3 complex matrix m (18 doubles, element (i, j) at 2 (i + 3 j);
+// the determinant does not depend on the storage transpose)
+__device__ __forceinline__ double2 det3(const double* m) {
+  double ar[9], ai[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    ar[k] = m[2 * k];
+    ai[k] = m[2 * k + 1];
+  }
+  // cofactor expansion along the first index: sum_j a(0,j) C(0,j)
+  auto mulr = [](double xr, double xi, double yr, double yi) { return xr * yr - xi * yi; };
+  auto muli = [](double xr, double xi, double yr, double yi) { return xr * yi + xi * yr; };
+  // element (i, j) at k = i + 3 j
+  double dr = 0, di = 0;
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    // minor (1, j1)(2, j2) - (1, j2)(2, j1)
+    const double pr = mulr(ar[1 + 3 * j1], ai[1 + 3 * j1], ar[2 + 3 * j2], ai[2 + 3 * j2]) -
+                      mulr(ar[1 + 3 * j2], ai[1 + 3 * j2], ar[2 + 3 * j1], ai[2 + 3 * j1]);
+    const double pi = muli(ar[1 + 3 * j1], ai[1 + 3 * j1], ar[2 + 3 * j2], ai[2 + 3 * j2]) -
+                      muli(ar[1 + 3 * j2], ai[1 + 3 * j2], ar[2 + 3 * j1], ai[2 + 3 * j1]);
+    dr += mulr(ar[3 * j], ai[3 * j], pr, pi);
+    di += muli(ar[3 * j], ai[3 * j], pr, pi);
+  }
+  return make_double2(dr, di);
+}
+
+// 4 threads: det of the reference link of each direction; also clears dev
+__global__ void k_link_det_ref(const double* __restrict__ aos, Geom g, double* __restrict__ ref,
+                               unsigned* __restrict__ dev) {
+  const int mu = threadIdx.x;
+  if (mu >= 4) return;
+  dev[2 * mu] = 0u;
+  dev[2 * mu + 1] = 0u;
+  int x[4] = {0, 0, 0, 0};
+  x[mu] = g.L[mu] - 1;
+  const int p = (x[0] + x[1] + x[2] + x[3]) & 1;
+  const int64_t site_cb = (x[0] >> 1) + (int64_t)(g.L[0] >> 1) * (x[1] + g.L[1] * (x[2] + (int64_t)g.L[2] * x[3]));
+  const double2 d = det3(aos + 72 * ((int64_t)p * g.Vcb + site_cb) + 18 * mu);
+  ref[2 * mu] = d.x;
+  ref[2 * mu + 1] = d.y;
+}
+
+// one thread per link (parity, site_cb, mu): the largest deviation of det U
+// from 1 (interior) or from the reference (last slice), per direction
+__global__ void k_link_det_dev(const double* __restrict__ aos, Geom g, const double* __restrict__ ref,
+                               unsigned* __restrict__ dev) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8 * (int64_t)g.Vcb) return;
+  const int mu = (int)(t & 3);
+  const int64_t ps = t >> 2;   // parity * Vcb + site_cb
+  const int p = ps >= g.Vcb ? 1 : 0;
+  const int64_t site_cb = ps - (int64_t)p * g.Vcb;
+  const int lh = g.L[0] >> 1;
+  int64_t r = site_cb / lh;
+  const int xh = (int)(site_cb - r * lh);
+  int x[4];
+  x[1] = (int)(r % g.L[1]);
+  r /= g.L[1];
+  x[2] = (int)(r % g.L[2]);
+  x[3] = (int)(r / g.L[2]);
+  x[0] = 2 * xh + ((x[1] + x[2] + x[3] + p) & 1);
+  const bool last = x[mu] == g.L[mu] - 1;
+  const double2 d = det3(aos + 72 * ps + 18 * mu);
+  const double er = last ? ref[2 * mu] : 1.0, ei = last ? ref[2 * mu + 1] : 0.0;
+  float e = (float)sqrt((d.x - er) * (d.x - er) + (d.y - ei) * (d.y - ei));
+  if (!(e <= 3.0e38f)) e = 3.0e38f;   // NaN or inf: the largest finite value
+  atomicMax(&dev[2 * mu + (last ? 1 : 0)], __float_as_uint(e));
+}
+
+hipError_t link_det_check(const double* aos, const Geom& g, double* ref, unsigned* dev, hipStream_t st) {
+  hipLaunchKernelGGL(k_link_det_ref, dim3(1), dim3(64), 0, st, aos, g, ref, dev);
+  const int64_t n = (int64_t)8 * g.Vcb;
+  hipLaunchKernelGGL(k_link_det_dev, dim3(nblocks(n, 256)), dim3(256), 0, st, aos, g, (const double*)ref, dev);
+  return hipGetLastError();
+}
+
+__global__ void k_flag_clear(unsigned* flag) {
+  if (threadIdx.x == 0) *flag = 0u;
+}
+
+__global__ void k_words_differ(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b, int64_t n,
+                               unsigned* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  bool d = false;
+  for (int64_t k = i; k < n; k += stride) d |= a[k] != b[k];
+  if (__any(d) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+hipError_t words_differ(const uint64_t* a, const uint64_t* b, int64_t n, unsigned* flag, hipStream_t st) {
+  hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(64), 0, st, flag);
+  const int64_t nb = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_words_differ, dim3((unsigned)std::max<int64_t>(nb, 1)), dim3(256), 0, st, a, b, n, flag);
   return hipGetLastError();
 }
 

@@ -559,7 +559,7 @@ static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, 
   P.nsites_out = L.g.Vcb;
   P.out_parity = target;
   P.block = pick_block((long long)L.g.Vcb * L.ls);
-  P.link_bc_sign = (hprec != PREC_D_D18);
+  if (int rc = L.set_hop_phase(P, hprec)) return rc;
   P.scale = scale;
   P.in = in;
   P.out = out;

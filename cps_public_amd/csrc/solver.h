@@ -70,6 +70,18 @@ struct Lattice {
   bool gauge_valid[NLINK];
   bool has_gauge = false;
   long long gauge_generation = 0;
+  // Determinant of the resident links, measured at import (link_det_check):
+  // link_phase[mu] is det U of the links on the last mu slice (1 periodic,
+  // -1 antiperiodic, e^{3 i theta} twisted by e^{i theta}); recon_ok says
+  // every interior link has det 1 and every boundary link of direction mu
+  // det link_phase[mu] (to 1e-5), the condition under which a link is
+  // rebuilt from its first two rows (12-real storage).
+  double link_phase[4][2];
+  bool recon_ok = false;
+  DevBuf det_scratch;
+  DevBuf gauge_stage;   // a re-import lands here and is compared with gauge_aos
+  int measure_link_phase();
+  int set_hop_phase(HopParams& P, int hprec) const;
   std::vector<Field> scratch;
   DevBuf partial, scal, io, io2;
   DevBuf nbr, nbr_neg;   // neighbour table of the geometry (hop kernels)
@@ -94,6 +106,7 @@ struct Lattice {
   Lattice();
   ~Lattice();
   int init(const int node_sites[4], int ls, const int bc[4]);
+  int set_twist(const double angle[4]);   // twisted directions' phase (CANONICAL import)
   int import_gauge(const double* u_host);
   int import_gauge_device(const double* u_dev);   // same, from device memory
   int import_gauge_from(const double* u, hipMemcpyKind kind);
@@ -198,7 +211,7 @@ int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, con
            double scale, const FieldRef& X, double xcoef, double* partial, int nrhs, const HopClover* hc);
 int hop_blocks_per_rhs(Lattice& L, const cps_hip_op* op);
 int tuned_launch(Lattice& L, HopParams& P);
-void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp_store, int* hprec);
+void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, bool recon_ok, int* sp_store, int* hprec);
 int apply_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, const FieldRef& t,
             int dag, double* partial, int nrhs);
 // t1_ready (Moebius): t1 already holds b in + c P in, the first step of M

@@ -72,7 +72,12 @@ int Lattice::init(const int node_sites[4], int ls_, const int bc[4]) {
   for (int mu = 0; mu < 4; mu++) {
     if (node_sites[mu] <= 0 || (node_sites[mu] & 1)) return CPS_HIP_ERR_ARG;
     g.L[mu] = node_sites[mu];
-    g.bc[mu] = bc ? (bc[mu] ? 1 : 0) : 0;
+    g.bc[mu] = bc ? bc[mu] : 0;
+    // BND_CND_PRD, BND_CND_APRD, BND_CND_TWISTED (enum.x:271-277); the
+    // G-parity kinds need the second flavour's field and are not supported
+    if (g.bc[mu] < 0 || g.bc[mu] > 2) return CPS_HIP_ERR_ARG;
+    g.bf[mu][0] = g.bc[mu] == 1 ? -1.0 : 1.0;   // a twist starts at angle 0
+    g.bf[mu][1] = 0.0;
   }
   if (ls_ < 1) return CPS_HIP_ERR_ARG;
   ls = ls_;
@@ -91,9 +96,20 @@ int Lattice::init(const int node_sites[4], int ls_, const int bc[4]) {
   return 0;
 }
 
+// GJP::TwistPhase (gjp.C:475-493): e^{-i pi theta}, theta in units of pi
+int Lattice::set_twist(const double angle[4]) {
+  for (int mu = 0; mu < 4; mu++) {
+    if (g.bc[mu] != 2) continue;
+    const double th = angle ? angle[mu] : 0.0;
+    g.bf[mu][0] = std::cos(M_PI * th);
+    g.bf[mu][1] = -std::sin(M_PI * th);
+  }
+  return 0;
+}
+
 long long Lattice::device_bytes() const {
   long long b = (long long)gauge_aos.bytes + partial.bytes + scal.bytes + io.bytes + io2.bytes + nbr.bytes +
-                nbr_neg.bytes;
+                nbr_neg.bytes + det_scratch.bytes + gauge_stage.bytes;
   for (int k = 0; k < NLINK; k++) b += gauge[k].bytes;
   for (auto& f : scratch) b += f.buf.bytes + f.nrm.bytes;
   for (int k = 0; k < NCLOVER; k++) b += clover[k].bytes;
@@ -104,20 +120,85 @@ int Lattice::import_gauge(const double* u_host) { return import_gauge_from(u_hos
 
 int Lattice::import_gauge_device(const double* u_dev) { return import_gauge_from(u_dev, hipMemcpyDeviceToDevice); }
 
+// A field equal (bit for bit) to the resident one keeps every copy derived
+// from it (low-precision links, clover sets): callers that pass the gauge
+// with every operator call, as CPS's library functions take it, pay one
+// transfer and a device comparison instead of a repack, and a clover field
+// built earlier stays valid.
 int Lattice::import_gauge_from(const double* u, hipMemcpyKind kind) {
   const size_t n = (size_t)72 * 2 * g.Vcb;
-  int rc = gauge_aos.ensure(n * sizeof(double));
-  if (rc) return rc;
-  CPSHIP_CHECK(hipMemcpyAsync(gauge_aos.p, u, n * sizeof(double), kind, stream));
+  int rc;
+  if (has_gauge && gauge_aos.bytes >= n * sizeof(double)) {
+    if ((rc = gauge_stage.ensure(n * sizeof(double)))) return rc;
+    if ((rc = det_scratch.ensure(8 * sizeof(double) + 9 * sizeof(unsigned)))) return rc;
+    unsigned* diff = (unsigned*)((double*)det_scratch.p + 8) + 8;
+    CPSHIP_CHECK(hipMemcpyAsync(gauge_stage.p, u, n * sizeof(double), kind, stream));
+    CPSHIP_CHECK(words_differ((const uint64_t*)gauge_stage.p, (const uint64_t*)gauge_aos.p, (int64_t)n, diff, stream));
+    unsigned h_diff = 1;
+    CPSHIP_CHECK(hipMemcpyAsync(&h_diff, diff, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    CPSHIP_CHECK(hipStreamSynchronize(stream));
+    if (!h_diff) return 0;
+    std::swap(gauge_stage.p, gauge_aos.p);
+    std::swap(gauge_stage.bytes, gauge_aos.bytes);
+  } else {
+    if ((rc = gauge_aos.ensure(n * sizeof(double)))) return rc;
+    CPSHIP_CHECK(hipMemcpyAsync(gauge_aos.p, u, n * sizeof(double), kind, stream));
+  }
   for (int k = 0; k < NLINK; k++) gauge_valid[k] = false;
   // a clover field built from the previous gauge field is stale
   n_clover = 0;
   for (int k = 0; k < NCLOVER; k++) clover_valid[k] = false;
   rc = ensure_gauge(LINK_D18);
   if (rc) return rc;
-  CPSHIP_CHECK(hipStreamSynchronize(stream));
+  if ((rc = measure_link_phase())) return rc;   // synchronises the stream
   has_gauge = true;
   gauge_generation++;
+  return 0;
+}
+
+// Boundary factor of the resident links, from the data itself: the FP64
+// links carry whatever BondCond applied (CPS's DiracOp constructor, or the
+// CANONICAL import), so the 12-real formats never depend on the handle's bc.
+int Lattice::measure_link_phase() {
+  int rc = det_scratch.ensure(8 * sizeof(double) + 9 * sizeof(unsigned));
+  if (rc) return rc;
+  double* ref = (double*)det_scratch.p;
+  unsigned* dev = (unsigned*)(ref + 8);
+  CPSHIP_CHECK(link_det_check((const double*)gauge_aos.p, g, ref, dev, stream));
+  double h_ref[8];
+  unsigned h_dev[8];
+  CPSHIP_CHECK(hipMemcpyAsync(h_ref, ref, sizeof(h_ref), hipMemcpyDeviceToHost, stream));
+  CPSHIP_CHECK(hipMemcpyAsync(h_dev, dev, sizeof(h_dev), hipMemcpyDeviceToHost, stream));
+  CPSHIP_CHECK(hipStreamSynchronize(stream));
+  const double tol = 1e-5;
+  recon_ok = true;
+  for (int mu = 0; mu < 4; mu++) {
+    float e0, e1;
+    std::memcpy(&e0, &h_dev[2 * mu], 4);
+    std::memcpy(&e1, &h_dev[2 * mu + 1], 4);
+    double pr = h_ref[2 * mu], pi = h_ref[2 * mu + 1];
+    const double m = std::sqrt(pr * pr + pi * pi);
+    if (!(e0 <= tol && e1 <= tol && std::fabs(m - 1.0) <= tol)) recon_ok = false;
+    // the exact value for the periodic and antiperiodic cases, else the
+    // measured phase on the unit circle
+    if (std::fabs(pr - 1.0) <= tol && std::fabs(pi) <= tol) pr = 1.0, pi = 0.0;
+    else if (std::fabs(pr + 1.0) <= tol && std::fabs(pi) <= tol) pr = -1.0, pi = 0.0;
+    else if (m > 0) pr /= m, pi /= m;
+    link_phase[mu][0] = pr;
+    link_phase[mu][1] = pi;
+  }
+  return 0;
+}
+
+// The boundary factors of a hop reading 12-real links; such links need the
+// resident field to be SU(3) times the boundary factor (recon_ok).
+int Lattice::set_hop_phase(HopParams& P, int hprec) const {
+  const bool twelve = hprec == PREC_F_F12 || hprec == PREC_H_H12 || hprec == PREC_H_B12;
+  if (twelve && !recon_ok) return CPS_HIP_ERR_ARG;
+  for (int mu = 0; mu < 4; mu++) {
+    P.bph[mu][0] = (float)link_phase[mu][0];
+    P.bph[mu][1] = (float)link_phase[mu][1];
+  }
   return 0;
 }
 
@@ -266,7 +347,7 @@ int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, con
   P.out_cb5 = target;
   P.ls = L.ls;
   P.block = pick_block((long long)o.nsites * o.nb);
-  P.link_bc_sign = (hprec != PREC_D_D18);
+  if ((rc = L.set_hop_phase(P, hprec))) return rc;
   P.two_over_a5 = 2.0 * op->a5_inv;
   P.neg_mass_two_over_a5 = -2.0 * op->mass * op->a5_inv;
   P.scale = scale;
@@ -830,12 +911,15 @@ int cg_double_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int
 // stored 18-real when a site carries at least 4 spinors (DWF slots, Moebius
 // slices, right-hand sides): the links are then a small share of the traffic
 // and skipping the third-row reconstruction saves arithmetic; otherwise the
-// 12-real form saves bandwidth.
-void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp_store, int* hprec) {
+// 12-real form saves bandwidth.  The 12-real form needs links that are SU(3)
+// times the boundary factor (recon_ok, measured at import); without it the
+// half-precision modes keep 18 reals, and MIXED_SINGLE (FP32 links exist only
+// 12-real) is refused by the hop (CPS_HIP_ERR_ARG).
+void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, bool recon_ok, int* sp_store, int* hprec) {
   int per_site = nrhs;
   if (op->kind == CPS_HIP_OP_DWF) per_site *= ls / 2;
   if (op->kind == CPS_HIP_OP_MOBIUS) per_site *= ls;
-  const bool full = per_site >= 4;
+  const bool full = per_site >= 4 || !recon_ok;
   switch (prec) {
     case CPS_HIP_PREC_DOUBLE: *sp_store = STORE_D; *hprec = PREC_D_D18; break;
     case CPS_HIP_PREC_MIXED_SINGLE: *sp_store = STORE_F; *hprec = PREC_F_F12; break;
@@ -861,8 +945,8 @@ static void sloppy_vector_storage(int sps, int* xst, int* rst) {
   else if (std::strcmp(e, "xhalf") == 0) { *xst = STORE_H; *rst = STORE_F; }
 }
 
-static void mixed_formats(int prec, const cps_hip_op* op, int ls, int nrhs, int* sp_store, int* hprec) {
-  precision_formats(prec, op, ls, nrhs, sp_store, hprec);
+static void mixed_formats(const Lattice& L, int prec, const cps_hip_op* op, int nrhs, int* sp_store, int* hprec) {
+  precision_formats(prec, op, L.ls, nrhs, L.recon_ok, sp_store, hprec);
 }
 
 // After a reliable update (rr_true = FP64 |src - MdagM x|^2 per rhs, the
@@ -909,7 +993,7 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   int rc = op_layout(op, L.ls, &o, L.g);
   if (rc) return rc;
   int sps, hprec;
-  mixed_formats(arg->precision, op, L.ls, nrhs, &sps, &hprec);
+  mixed_formats(L, arg->precision, op, nrhs, &sps, &hprec);
   const int64_t n = o.per_rhs * nrhs;
   Field& r = L.field(S_R, STORE_D, n);
   Field& ap = L.field(S_AP, STORE_D, n);

@@ -36,21 +36,41 @@ def ferm_site_perm(L):
     return _cb_index(L, x, y, z, t) + vcb * par
 
 
-def bond_cond(u_canon, L, bc):
-    """Toggle the sign of the mu-links on the last slice of every
-    antiperiodic direction (CANONICAL order)."""
-    u = u_canon.reshape(L[3], L[2], L[1], L[0], 4, 18).copy()
+def boundary_factor(bc, twist=None):
+    """Factor Lattice::BondCond applies to the last slice of each direction
+    (lattice_base.C:5313-5369): 1 periodic, -1 antiperiodic, GJP::TwistPhase
+    e^{-i pi theta} twisted (gjp.C:475-493; theta = twist[mu] in units of pi)."""
+    f = []
     for mu in range(4):
-        if not bc[mu]:
-            continue
-        if mu == 0:
-            u[:, :, :, L[0] - 1, 0, :] *= -1
-        elif mu == 1:
-            u[:, :, L[1] - 1, :, 1, :] *= -1
-        elif mu == 2:
-            u[:, L[2] - 1, :, :, 2, :] *= -1
+        if bc[mu] == 1:
+            f.append(-1.0 + 0j)
+        elif bc[mu] == 2:
+            th = 0.0 if twist is None else float(twist[mu])
+            f.append(complex(np.cos(np.pi * th), -np.sin(np.pi * th)))
         else:
-            u[L[3] - 1, :, :, :, 3, :] *= -1
+            f.append(1.0 + 0j)
+    return f
+
+
+def bond_cond(u_canon, L, bc, twist=None):
+    """Multiply the mu-links on the last slice of every non-periodic
+    direction by its boundary factor (CANONICAL order; real fields stay real
+    for periodic / antiperiodic conditions)."""
+    fac = boundary_factor(bc, twist)
+    u = u_canon.reshape(L[3], L[2], L[1], L[0], 4, 9, 2).copy()
+    sl = {0: (slice(None), slice(None), slice(None), L[0] - 1),
+          1: (slice(None), slice(None), L[1] - 1, slice(None)),
+          2: (slice(None), L[2] - 1, slice(None), slice(None)),
+          3: (L[3] - 1, slice(None), slice(None), slice(None))}
+    for mu in range(4):
+        f = fac[mu]
+        if f == 1:
+            continue
+        blk = u[sl[mu] + (mu,)]
+        re, im = blk[..., 0].copy(), blk[..., 1].copy()
+        blk[..., 0] = f.real * re - f.imag * im
+        blk[..., 1] = f.real * im + f.imag * re
+        u[sl[mu] + (mu,)] = blk
     return u.reshape(-1)
 
 

@@ -15,7 +15,7 @@
  *                             cb at 4-D parity (s+cb)%2
  *   5-D Moebius (4-D prec.) : Ls slices of V/2 sites of one 4-D parity
  *   gauge                   : [parity][site_cb][mu][col][row][re,im]
- *                             (boundary signs already applied, as the
+ *                             (boundary factors already applied, as the
  *                             DiracOp constructor leaves it)
  * Multi-RHS variants take nrhs such vectors back to back.
  *
@@ -52,8 +52,13 @@ extern "C" {
 
 /* solver precision modes */
 #define CPS_HIP_PREC_DOUBLE 0        /* plain FP64 CG (the CPS algorithm) */
-#define CPS_HIP_PREC_MIXED_HALF 1    /* FP32 math, FP16 spinors + FP16 12-real links */
-#define CPS_HIP_PREC_MIXED_BF16 2    /* FP32 math, FP16 spinors + BF16 12-real links */
+/* Link storage of the sloppy operator: FP16/BF16 links are kept 18-real when
+ * a site carries at least 4 spinors (DWF slots x rhs, Moebius slices x rhs,
+ * rhs) and 12-real (third row rebuilt) otherwise; FP32 links are always
+ * 12-real.  12-real links need a field that is SU(3) times the boundary
+ * factor (checked at import, cps_hip_gauge_boundary_phase). */
+#define CPS_HIP_PREC_MIXED_HALF 1    /* FP32 math, FP16 spinors + FP16 links (18- or 12-real) */
+#define CPS_HIP_PREC_MIXED_BF16 2    /* FP32 math, FP16 spinors + BF16 links (18- or 12-real) */
 #define CPS_HIP_PREC_MIXED_SINGLE 3  /* FP32 math, FP32 spinors + FP32 12-real links */
 
 typedef struct cps_hip_lattice cps_hip_lattice;
@@ -96,22 +101,40 @@ typedef struct {
 /* ---- lattice handle ------------------------------------------------------
  * Replaces wilson_init/dwf_init (wilson_init.C:52-112, dwf_init.C:41-99): it
  * holds the local geometry, the device scratch fields and the resident gauge.
- * bc[mu] = 1 marks an antiperiodic direction (GJP.Bc(mu) == BND_CND_APRD);
- * the sign itself travels in the gauge field as in CPS, and bc is used only
- * to rebuild the third row of compressed (12-real) links.  ls may be 1 for
- * 4-D operators. */
+ * bc[mu] is GJP.Bc(mu) as CPS's BndCndType (enum.x:271-277): 0 periodic,
+ * 1 antiperiodic, 2 twisted (the G-parity kinds are refused).  bc is used
+ * only by cps_hip_gauge_import_canonical, which applies the boundary factor
+ * the way Lattice::BondCond does; a WILSON-order field passed to
+ * cps_hip_gauge_import or to the library calls carries its factors already
+ * (as the DiracOp constructor leaves it), so a handle for such fields may
+ * pass bc = {0,0,0,0}.  ls may be 1 for 4-D operators. */
 int cps_hip_lattice_create(cps_hip_lattice **lat, const int node_sites[4], int ls, const int bc[4]);
 int cps_hip_lattice_destroy(cps_hip_lattice *lat);
+/* Twist angles theta[mu] (GJP.TwistAngle, in units of pi) of the directions
+ * created with bc = 2: the CANONICAL import multiplies their last slice by
+ * GJP.TwistPhase = e^{-i pi theta} (gjp.C:475-493, lattice_base.C:5327-5349).
+ * Takes effect at the next cps_hip_gauge_import_canonical. */
+int cps_hip_lattice_set_twist(cps_hip_lattice *lat, const double twist_angle[4]);
 /* Upload the WILSON-order FP64 gauge field (72*V/2*2 reals) and build the
- * device copies (FP64 18-real, and the 12-real low-precision copies on first
- * use). */
+ * device copies (FP64 18-real, and the low-precision copies on first use).
+ * The boundary factor of every direction is measured from the field itself
+ * (det U of its last-slice links): the 12-real formats rebuild the third row
+ * as det(U) conj(row0 x row1), so they never depend on the handle's bc.  A
+ * field bit-identical to the resident one is compared on the device and
+ * changes nothing (derived copies and clover sets stay valid). */
 int cps_hip_gauge_import(cps_hip_lattice *lat, const double *u_wilson);
+/* The boundary factors measured at the last import: phase[2 mu], phase[2 mu
+ * + 1] = det U (re, im) of the last-slice mu-links; *recon_ok (may be NULL)
+ * = 1 if every interior link has det 1 and every last-slice link of each
+ * direction the same det, to 1e-5 (required by the 12-real formats). */
+int cps_hip_gauge_boundary_phase(const cps_hip_lattice *lat, double phase[8], int *recon_ok);
 /* Device bytes currently held by the handle. */
 long long cps_hip_lattice_device_bytes(const cps_hip_lattice *lat);
 
 /* ---- Wilson library (wilson.h:187-284) ---------------------------------
  * u: the WILSON-order gauge; if non-NULL it is imported first (equivalent to
- * cps_hip_gauge_import), if NULL the resident gauge is used. */
+ * cps_hip_gauge_import: an unchanged field costs one transfer and a device
+ * comparison), if NULL the resident gauge is used. */
 /* wilson_dslash (wilson_dslash.C:173-515): chi on parity 1-cb from psi on cb */
 int cps_hip_wilson_dslash(cps_hip_lattice *lat, double *chi, const double *u, const double *psi, int cb, int dag);
 /* wilson_m / wilson_mdag (wilson_m.C:49-93, wilson_mdag.C:50-95) */
@@ -182,9 +205,10 @@ int cps_hip_matinv(cps_hip_lattice *lat, const cps_hip_op *op, double *out, cons
 /* ---- CANONICAL order (Lattice::Convert / Fconvert on the device) ----------
  * cps_hip_gauge_import_canonical: u is the CANONICAL gauge field
  * [t][z][y][x][mu][row][col][re,im] (lattice_base.C:5350-5351) WITHOUT
- * boundary signs; the import transposes it to the WILSON order
- * (convert_func.C:318-415) and applies the signs of the handle's bc
- * (Lattice::BondCond, lattice_base.C:5313-5369), replacing what the DiracOp
+ * boundary factors; the import transposes it to the WILSON order
+ * (convert_func.C:318-415) and multiplies the last slice of each direction by
+ * the factor of the handle's bc (-1 antiperiodic, e^{-i pi theta} twisted;
+ * Lattice::BondCond, lattice_base.C:5313-5369), replacing what the DiracOp
  * constructor does to the Lattice's field (dirac_op_base.C:164,
  * d_op_wilson.C:118-121). */
 int cps_hip_gauge_import_canonical(cps_hip_lattice *lat, const double *u_canon);

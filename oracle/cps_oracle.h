@@ -42,6 +42,9 @@ void ora_ferm5_canon_to_4dprec(double *dst, const double *src, const int L[4], i
 void ora_ferm5_4dprec_to_canon(double *dst, const double *src, const int L[4], int ls);
 /* bc[mu] = 1 for anti-periodic; toggles the sign of links with x_mu = L_mu-1 */
 void ora_bond_cond(double *u_canon, const int L[4], const int bc[4]);
+/* bc[mu] = 1 antiperiodic (x -1), 2 twisted (x e^{-i pi twist[mu]}, its
+ * conjugate with undo = 1) on the links with x_mu = L_mu-1 */
+void ora_bond_cond_twist(double *u_canon, const int L[4], const int bc[4], const double twist[4], int undo);
 
 /* ---- Wilson ------------------------------------------------------------ */
 void ora_wilson_dslash(double *chi, const double *u, const double *psi,

@@ -166,8 +166,9 @@ void ora_ferm5_4dprec_to_canon(double *dst, const double *src, const int L[4],
   free(slice);
 }
 
-/* Lattice::BondCond, lattice_base.C:5313-5369 (APRD case only): multiply the
- * mu-link of every site with x_mu = L_mu - 1 by -1.  CANONICAL order. */
+/* Lattice::BondCond, lattice_base.C:5313-5369, antiperiodic directions:
+ * multiply the mu-link of every site with x_mu = L_mu - 1 by -1.  CANONICAL
+ * order.  (Twisted directions: ora_bond_cond_twist.) */
 void ora_bond_cond(double *u, const int L[4], const int bc[4]) {
   int mu, x[4], i;
   for (mu = 0; mu < 4; mu++) {
@@ -179,6 +180,43 @@ void ora_bond_cond(double *u, const int L[4], const int bc[4]) {
             if (x[mu] != L[mu] - 1) continue;
             int off = mu + 4 * (x[0] + L[0] * (x[1] + L[1] * (x[2] + L[2] * x[3])));
             for (i = 0; i < 18; i++) u[18 * off + i] = -u[18 * off + i];
+          }
+  }
+}
+
+/* Lattice::BondCond, lattice_base.C:5313-5369, with the twisted kind: for
+ * bc[mu] = 1 (BND_CND_APRD) the last-slice mu-links are multiplied by -1,
+ * for bc[mu] = 2 (BND_CND_TWISTED) by GJP::TwistPhase(mu) = cos(pi theta) -
+ * i sin(pi theta) (gjp.C:475-493, theta = twist[mu] in units of pi), or by
+ * its conjugate when undo = 1 (the odd calls of :5327-5331, which remove the
+ * twist again).  CANONICAL order. */
+void ora_bond_cond_twist(double *u, const int L[4], const int bc[4], const double twist[4], int undo) {
+  const double pi = 3.1415926535897932384626433832795;
+  int mu, x[4], i;
+  for (mu = 0; mu < 4; mu++) {
+    double fr, fi;
+    if (bc[mu] == 1) {
+      fr = -1.0;
+      fi = 0.0;
+    } else if (bc[mu] == 2) {
+      fr = cos(twist[mu] * pi);
+      fi = -sin(twist[mu] * pi);
+      if (undo) fi = -fi;
+    } else {
+      continue;
+    }
+    for (x[3] = 0; x[3] < L[3]; x[3]++)
+      for (x[2] = 0; x[2] < L[2]; x[2]++)
+        for (x[1] = 0; x[1] < L[1]; x[1]++)
+          for (x[0] = 0; x[0] < L[0]; x[0]++) {
+            if (x[mu] != L[mu] - 1) continue;
+            int off = mu + 4 * (x[0] + L[0] * (x[1] + L[1] * (x[2] + L[2] * x[3])));
+            double *m = u + 18 * off;
+            for (i = 0; i < 9; i++) {
+              double a = m[2 * i], b = m[2 * i + 1];
+              m[2 * i] = fr * a - fi * b;
+              m[2 * i + 1] = fr * b + fi * a;
+            }
           }
   }
 }

@@ -61,6 +61,7 @@ def _declare(l):
         "ora_ferm5_canon_to_4dprec": [vp, vp, vp, ctypes.c_int],
         "ora_ferm5_4dprec_to_canon": [vp, vp, vp, ctypes.c_int],
         "ora_bond_cond": [vp, vp, vp],
+        "ora_bond_cond_twist": [vp, vp, vp, vp, ctypes.c_int],
         "ora_wilson_dslash": [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp],
         "ora_wilson_m": [vp, vp, vp, ctypes.c_double, vp, vp],
         "ora_wilson_mdag": [vp, vp, vp, ctypes.c_double, vp, vp],
@@ -168,6 +169,13 @@ def ferm5_4dprec_to_canon(f, L, ls):
 def bond_cond(u_canon, L, bc):
     u = u_canon.copy()
     lib().ora_bond_cond(_p(u), _L(L), _L(bc))
+    return u
+
+
+def bond_cond_twist(u_canon, L, bc, twist, undo=False):
+    u = u_canon.copy()
+    tw = np.ascontiguousarray(np.asarray(twist, dtype=np.float64))
+    lib().ora_bond_cond_twist(_p(u), _L(L), _L(bc), _p(tw), int(bool(undo)))
     return u
 
 

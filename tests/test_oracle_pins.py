@@ -192,6 +192,59 @@ def test_bond_cond(oracle, gauge):
     assert np.array_equal(LY.bond_cond(u, L, [0, 0, 0, 1]), b)
 
 
+def test_bond_cond_twisted(oracle, gauge):
+    """Lattice::BondCond with twisted directions (lattice_base.C:5313-5369):
+    the last-slice mu-links times GJP::TwistPhase = e^{-i pi theta}
+    (gjp.C:475-493), -1 for an antiperiodic direction; the odd call (undo)
+    applies the conjugate phase and restores the field."""
+    L = LW
+    u, U = gauge(L, 0.5, 4)
+    bc, tw = [2, 1, 0, 2], [0.3, 0.0, 0.0, -0.7]
+    b = oracle.bond_cond_twist(u, L, bc, tw)
+    Ub = T.links(b, L)
+    c = T.coords(L)
+    fac = np.ones((int(np.prod(L)), 4), dtype=complex)
+    fac[c[:, 0] == L[0] - 1, 0] = np.exp(-1j * np.pi * 0.3)
+    fac[c[:, 1] == L[1] - 1, 1] = -1
+    fac[c[:, 3] == L[3] - 1, 3] = np.exp(1j * np.pi * 0.7)
+    assert np.allclose(Ub, U * fac[:, :, None, None], rtol=0, atol=1e-15)
+    assert np.allclose(oracle.bond_cond_twist(b, L, bc, tw, undo=True), u, rtol=0, atol=1e-15)
+    assert np.allclose(LY.bond_cond(u, L, bc, tw), b, rtol=0, atol=1e-15)
+    # antiperiodic only: identical to the APRD routine
+    assert np.array_equal(oracle.bond_cond_twist(u, L, [0, 0, 0, 1], [0.0] * 4), oracle.bond_cond(u, L, [0, 0, 0, 1]))
+
+
+def test_wilson_plane_wave_twisted(oracle):
+    """U = 1 with every direction twisted by theta_mu: the plane wave with
+    p_mu = (2 pi n_mu - pi theta_mu) / L_mu satisfies the twisted periodicity
+    e^{i p_mu L_mu} = e^{-i pi theta_mu}, so D e^{ipx} chi = sum_mu (2 cos p_mu
+    - 2 i gamma_mu sin p_mu) e^{ipx} chi at every site."""
+    L = [4, 6, 4, 8]
+    u = F.unit_gauge(L)
+    tw = np.array([0.25, -0.4, 0.6, 1.0 / 3.0])
+    uw = oracle.gauge_canon_to_wilson(oracle.bond_cond_twist(u, L, [2, 2, 2, 2], tw), L)
+    c = T.coords(L)
+    n = np.array([1, 0, 3, 1])
+    p = (2 * np.pi * n - np.pi * tw) / np.array(L)
+    chi = np.array([0.3 + 0.1j, -0.2 + 0.5j, 0.7 - 0.4j, 0.1 + 0.9j])
+    col = np.array([1.0, -0.5j, 0.25])
+    phase = np.exp(1j * (c @ p))
+    psi = phase[:, None] * np.kron(chi, col)[None, :]
+    K = sum(2 * np.cos(p[m]) * np.eye(4) - 2j * np.sin(p[m]) * T.GAMMA[m] for m in range(4))
+    ref = phase[:, None] * np.kron(K @ chi, col)[None, :]
+    for cb in (0, 1):
+        got = oracle.wilson_dslash(uw, cb_from_canon(psi, L, cb), cb, 0, L)
+        assert relerr(got, cb_from_canon(ref, L, 1 - cb)) < TOL
+    # the untwisted momentum is not an eigenvector here (the test can fail)
+    p0 = 2 * np.pi * n / np.array(L)
+    ph0 = np.exp(1j * (c @ p0))
+    psi0 = ph0[:, None] * np.kron(chi, col)[None, :]
+    K0 = sum(2 * np.cos(p0[m]) * np.eye(4) - 2j * np.sin(p0[m]) * T.GAMMA[m] for m in range(4))
+    ref0 = ph0[:, None] * np.kron(K0 @ chi, col)[None, :]
+    got0 = oracle.wilson_dslash(uw, cb_from_canon(psi0, L, 0), 0, 0, L)
+    assert relerr(got0, cb_from_canon(ref0, L, 1)) > 1e-3
+
+
 def test_blas(oracle):
     """dotProduct (vector_util.C:727-740) and fTimesV1PlusV2 (:825-835)."""
     rng = np.random.default_rng(5)
