@@ -86,11 +86,12 @@ def op_for(cfg):
                      c=cfg.get("c", 0.0))
 
 
-def time_solves(lat, op, src_dev, arg, nrhs, warmup, steps):
+def time_solves(lat, op, src_dev, arg, nrhs, warmup, steps, keep=False):
     """Each step is one complete solve of all nrhs sources, which are resident
     in HBM (CPS layout, FP64) before the clock starts; the solutions are
     resident in HBM and their FP64 true residual computed when it stops
-    (cps_hip_invcg_device returns after a device synchronisation)."""
+    (cps_hip_invcg_device returns after a device synchronisation).  With
+    keep, the last solutions are returned (host copy, FP64) as well."""
     sol = C.DeviceBuffer(src_dev.nbytes)
     zeros = np.zeros(src_dev.nbytes // 8)
     res = None
@@ -105,63 +106,135 @@ def time_solves(lat, op, src_dev, arg, nrhs, warmup, steps):
         t.append(time.perf_counter() - t0)
         print(f"# solve: {t[-1]:.4f} s, iterations {[r.iterations for r in res]}, "
               f"true_res {max(r.true_res for r in res):.3e}", file=sys.stderr, flush=True)
+    x = sol.download() if keep else None
     sol.free()
-    return t, res
+    return (t, res, x) if keep else (t, res)
 
 
-def cpu_baseline(cfg, u_w, src, gpu_iters):
-    """The CPU restatement (oracle/) of the CPS noarch FP64 CG, one thread,
-    on the same gauge and source, for a bounded number of iterations; the
-    per-iteration time is scaled to the GPU FP64 iteration count."""
+def rel_err_per_rhs(x, ref, nrhs):
+    """|x - ref| / |ref| for each rhs (FP64 on the host)."""
+    n = x.size // nrhs
+    return [float(np.linalg.norm(x[k * n:(k + 1) * n] - ref[k * n:(k + 1) * n])
+                  / np.linalg.norm(ref[k * n:(k + 1) * n])) for k in range(nrhs)]
+
+
+def host_info():
+    """The host the CPU leg ran on: CPU model, logical CPUs this process may
+    use, OpenMP threads, memory."""
+    model, mem_kb = "unknown", 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    mem_kb = int(line.split()[1])
+                    break
+    except OSError:
+        pass
+    ncpu = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return {"cpu": model, "cpus_available": ncpu, "omp_num_threads": omp or None,
+            "mem_gb": round(mem_kb / 2 ** 20, 1)}
+
+
+def cpu_threads():
+    """Threads of the multi-threaded CPU leg: OMP_NUM_THREADS if set (the
+    machine's CPU share), else the CPUs this process may run on."""
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return omp if omp > 0 else len(os.sched_getaffinity(0))
+
+
+def _oracle_cg(O, cfg, u_w, s0, stop_rsd, max_iter, clov=None):
+    L, ls = cfg["L"], cfg["ls"]
+    if cfg["kind"] == C.OP_DWF:
+        return O.dwf_invcg(u_w, s0, cfg["mass"], stop_rsd, max_iter, O.dwf_struct(L, ls, cfg["m5"]))
+    if cfg["kind"] == C.OP_MOBIUS:
+        m = O.mobius_struct(L, ls, cfg["m5"], cfg["b"], cfg["c"])
+        return O.mobius_invcg(u_w, s0, cfg["mass"], stop_rsd, max_iter, m)
+    if cfg["kind"] == C.OP_CLOVER:
+        kappa, A_oo, A_ee_inv = clov
+        return O.clover_invcg(u_w, A_oo, A_ee_inv, s0, kappa, stop_rsd, max_iter, L)
+    return O.wilson_invcg(u_w, s0, cfg["mass"], stop_rsd, max_iter, L)
+
+
+def cpu_baseline(cfg, u_w, src, gpu_iters, full=False):
+    """The CPU restatement (oracle/) of the CPS noarch FP64 CG on the same
+    gauge field and rhs 0, on this machine's host cores: with OpenMP over the
+    sites of the hopping term at the machine's thread count (the reported
+    value) and with one thread (CPS noarch: wilson_dslash.C:280-477 has no
+    OpenMP).  full: to convergence at 1e-10 (C1, C2); otherwise a bounded
+    sample: a few CG iterations at full size, per-iteration time times the
+    GPU FP64 iteration count (a projection, labelled as such)."""
     from oracle import oracle as O
     O.build()
     L, ls = cfg["L"], cfg["ls"]
     per = 24 * (int(np.prod(L)) // 2) * ls
     s0 = np.ascontiguousarray(src[:per])
-    n_it = 3 if ls > 1 or np.prod(L) > 65536 else 50
-    big = np.prod(L) * ls > 4_000_000
-    if big and cfg["kind"] in (C.OP_MOBIUS, C.OP_DWF):
-        # one M^dag M of the oracle takes ~100 s single-threaded at this size:
-        # time it on a sub-lattice with Lt/8 time slices (same Ls and
-        # parameters, cost linear in the volume) and scale to the workload
+    nthr = cpu_threads()
+    clov = None
+    if cfg["kind"] == C.OP_CLOVER:
+        # the clover matrices are set-up work (CloverMatChkb), outside the clock
+        kappa = cfg["kappas"][0]
+        O.set_threads(nthr)
+        clov = (kappa, O.clover_mat_chkb(u_w, 1, 0, kappa, cfg["csw"], L),
+                O.clover_mat_chkb(u_w, 0, 1, kappa, cfg["csw"], L))
+    out = {"unit": "s/RHS", "kind": "port", "host": host_info()}
+    if full:
+        legs = {}
+        for thr in ([nthr, 1] if nthr > 1 else [1]):
+            O.set_threads(thr)
+            t0 = time.perf_counter()
+            _, it, tr = _oracle_cg(O, cfg, u_w, s0, 1e-10, 100000, clov)
+            el = time.perf_counter() - t0
+            legs[thr] = {"value": el, "cores": thr, "iterations": it, "true_res": tr}
+            print(f"# cpu leg: {thr} threads, {it} iterations, {el:.1f} s, true_res {tr:.2e}",
+                  file=sys.stderr, flush=True)
+        out.update(value=legs[nthr]["value"], cores=nthr,
+                   sample=f"rhs 0 solved to convergence (stop_rsd 1e-10) by the oracle FP64 CG with {nthr} "
+                          f"threads: {legs[nthr]['iterations']} iterations, true residual "
+                          f"{legs[nthr]['true_res']:.2e}")
+        if 1 in legs and nthr != 1:
+            out["one_thread"] = {"value": legs[1]["value"], "cores": 1,
+                                 "sample": f"the same solve with one thread: {legs[1]['iterations']} iterations"}
+        return out
+    # bounded sample at full size with nthr threads: n_it CG iterations plus
+    # the two extra M^dag M of InvCg (initial residual, true residual)
+    vol5 = int(np.prod(L)) * ls
+    n_it = 2 if vol5 > 4_000_000 else (5 if vol5 > 100_000 else 50)
+    O.set_threads(nthr)
+    t0 = time.perf_counter()
+    _, it, _ = _oracle_cg(O, cfg, u_w, s0, 1e-30, n_it + 1, clov)
+    el = time.perf_counter() - t0
+    per_iter = el / (n_it + 2)
+    out.update(value=per_iter * gpu_iters, cores=nthr,
+               sample=f"{n_it} CG iterations (+2 M^dag M) of the oracle FP64 CG on rhs 0 at full size with "
+                      f"{nthr} threads, {el:.1f} s; per-iteration time x {gpu_iters} iterations (the GPU FP64 "
+                      f"count): a projection, not a solve to convergence")
+    # one thread, as CPS noarch runs the hopping term: one M^dag M on a slab
+    # of Lt/8 time slices (same Ls and parameters; cost linear in volume)
+    if vol5 > 4_000_000 and cfg["kind"] in (C.OP_MOBIUS, C.OP_DWF):
         Lt_sub = max(2, L[3] // 8)
         Ls_ = [L[0], L[1], L[2], Lt_sub]
         u_sub = LY.gauge_canon_to_wilson(LY.bond_cond(F.gauge_field(Ls_, cfg["eps"], cfg["seed"]), Ls_, [0, 0, 0, 1]),
                                          Ls_)
         s_sub = F.gaussian_fermion(int(np.prod(Ls_)) // 2 * ls, cfg["seed"] + 1)
+        O.set_threads(1)
         t0 = time.perf_counter()
         if cfg["kind"] == C.OP_MOBIUS:
             O.mobius_mdagm(u_sub, s_sub, cfg["mass"], O.mobius_struct(Ls_, ls, cfg["m5"], cfg["b"], cfg["c"]))
         else:
             O.dwf_mdagm(u_sub, s_sub, cfg["mass"], O.dwf_struct(Ls_, ls, cfg["m5"]))
-        el = (time.perf_counter() - t0) * L[3] / Lt_sub
-        return {"value": el * gpu_iters, "unit": "s/RHS", "cores": 1, "kind": "port",
-                "sample": f"one M^dag M of the oracle (FP64, one thread) on a {L[0]}x{L[1]}x{L[2]}x{Lt_sub} "
-                          f"sub-lattice with Ls {ls} and the same parameters, "
-                          f"{el * Lt_sub / L[3]:.1f} s, x {L[3] // Lt_sub} (volume) = {el:.1f} s per M^dag M; "
-                          f"x {gpu_iters} iterations (the GPU FP64 count; CG vector updates not counted)"}
-    if cfg["kind"] == C.OP_CLOVER:
-        # the clover matrices are set-up work (CloverMatChkb), outside the clock
-        kappa = cfg["kappas"][0]
-        A_oo = O.clover_mat_chkb(u_w, 1, 0, kappa, cfg["csw"], L)
-        A_ee_inv = O.clover_mat_chkb(u_w, 0, 1, kappa, cfg["csw"], L)
-    t0 = time.perf_counter()
-    if cfg["kind"] == C.OP_DWF:
-        d = O.dwf_struct(L, ls, cfg["m5"])
-        _, it, _ = O.dwf_invcg(u_w, s0, cfg["mass"], 1e-30, n_it + 1, d)
-    elif cfg["kind"] == C.OP_MOBIUS:
-        m = O.mobius_struct(L, ls, cfg["m5"], cfg["b"], cfg["c"])
-        _, it, _ = O.mobius_invcg(u_w, s0, cfg["mass"], 1e-30, n_it + 1, m)
-    elif cfg["kind"] == C.OP_CLOVER:
-        _, it, _ = O.clover_invcg(u_w, A_oo, A_ee_inv, s0, kappa, 1e-30, n_it + 1, L)
-    else:
-        _, it, _ = O.wilson_invcg(u_w, s0, cfg["mass"], 1e-30, n_it + 1, L)
-    el = time.perf_counter() - t0
-    # each oracle CG runs n_it iterations plus two extra MdagM (initial and true residual)
-    per_iter = el / (n_it + 2)
-    return {"value": per_iter * gpu_iters, "unit": "s/RHS", "cores": 1, "kind": "port",
-            "sample": f"{n_it} CG iterations (+2 MdagM) of the oracle FP64 CG on rhs 0 of the same workload, "
-                      f"{el:.1f} s; per-iteration time x {gpu_iters} iterations (the GPU FP64 count)"}
+        el1 = (time.perf_counter() - t0) * L[3] / Lt_sub
+        out["one_thread"] = {"value": el1 * gpu_iters, "cores": 1,
+                             "sample": f"one M^dag M of the oracle with one thread on a {L[0]}x{L[1]}x{L[2]}x"
+                                       f"{Lt_sub} slab, x {L[3] // Lt_sub} (volume) = {el1:.1f} s per M^dag M, x "
+                                       f"{gpu_iters} iterations (projection; CG vector updates not counted)"}
+    O.set_threads(nthr)
+    return out
 
 
 SERVE_LATTICES = [([8, 8, 8, 8], 8), ([16, 16, 16, 32], 16), ([24, 24, 24, 48], 16)]
@@ -256,6 +329,12 @@ def main():
                     help="c5: include the 24^3x48 lattice in the mix (slow in FP64; off by default to keep the "
                          "default run within minutes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU leg: solve rhs 0 to convergence with the machine's thread count and with one thread "
+                         "(instead of the bounded sample); meant for c1 and c2")
+    ap.add_argument("--cpu-only", action="store_true", help="run only the CPU leg (no GPU work)")
+    ap.add_argument("--no-bf16", action="store_true",
+                    help="c3h: skip the extra solve of the same workload with BF16 link storage")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
     ap.add_argument("--no-dirac-res", action="store_true",
                     help="skip the untimed solve that reports |xi - M_pc x| / |xi| for rhs 0")
@@ -280,6 +359,10 @@ def main():
     print(f"# {cfg['name']}: building inputs", file=sys.stderr, flush=True)
     u_w, src = make_inputs(cfg)
     print("# inputs ready", file=sys.stderr, flush=True)
+    if a.cpu_only:
+        print(json.dumps({"config": {"workload": cfg["name"]},
+                          "cpu_baseline": cpu_baseline(cfg, u_w, src, 0, full=True)}))
+        return
     lat = C.Lattice(cfg["L"], cfg["ls"], (0, 0, 0, 1))
     lat.import_gauge(u_w)
     if cfg["kind"] == C.OP_CLOVER:
@@ -302,19 +385,41 @@ def main():
     delta = a.reliable_delta if a.reliable_delta is not None else cfg.get("reliable_delta", 0.1)
     arg_main = C.make_cg_arg(1e-10, a.max_iter, precision=prec_main, reliable_every=100,
                              reliable_delta=delta)
-    t_main, res_main = time_solves(lat, op, src_dev, arg_main, nrhs, a.warmup, a.steps)
+    t_main, res_main, x_main = time_solves(lat, op, src_dev, arg_main, nrhs, a.warmup, a.steps, keep=True)
     ms = 1e3 * float(np.mean(t_main))
     s_per_rhs = float(np.mean(t_main)) / nrhs
 
     fp64 = None
+    x64 = None
     if mixed is not None and not a.no_fp64:
         # a few-iteration FP64 solve first, so that kernel autotuning is not timed
         time_solves(lat, op, src_dev, C.make_cg_arg(1e-10, 3, precision=C.PREC_DOUBLE), nrhs, 1, 0)
         arg64 = C.make_cg_arg(1e-10, a.max_iter, precision=C.PREC_DOUBLE)
-        t64, res64 = time_solves(lat, op, src_dev, arg64, nrhs, 0, 1)
+        t64, res64, x64 = time_solves(lat, op, src_dev, arg64, nrhs, 0, 1, keep=True)
         fp64 = {"s_per_rhs": float(np.mean(t64)) / nrhs, "iterations": [r.iterations for r in res64],
                 "true_res_max": max(r.true_res for r in res64)}
     gpu_fp64_iters = (fp64["iterations"][0] if fp64 else res_main[0].iterations)
+    # per-solve error of the mixed solution against the FP64 GPU CG solution
+    # of the same system (both to true residual 1e-10)
+    err64 = rel_err_per_rhs(x_main, x64, nrhs) if x64 is not None else None
+    if err64:
+        print(f"# |x_mixed - x_fp64| / |x_fp64| per rhs: max {max(err64):.2e}", file=sys.stderr, flush=True)
+
+    # the same workload with BF16 link storage (the link format of the
+    # Moebius workload as specified), one timed solve after a warm-up
+    bf16 = None
+    if a.config == "c3h" and not a.no_bf16 and fp64 is not None:
+        cb = CONFIGS["c3"]
+        arg_b = C.make_cg_arg(1e-10, a.max_iter, precision=C.PREC_MIXED_BF16, reliable_every=100,
+                              reliable_delta=cb.get("reliable_delta", 0.1))
+        time_solves(lat, op, src_dev, C.make_cg_arg(1e-10, 3, precision=C.PREC_MIXED_BF16), nrhs, 1, 0)
+        tb, resb, xb = time_solves(lat, op, src_dev, arg_b, nrhs, 0, 1, keep=True)
+        sb = float(np.mean(tb)) / nrhs
+        bf16 = {"workload": cb["name"], "s_per_rhs": sb, "speedup_vs_fp64": fp64["s_per_rhs"] / sb,
+                "iterations": [r.iterations for r in resb], "true_res_max": max(r.true_res for r in resb),
+                "reliable_delta": cb.get("reliable_delta", 0.1),
+                "err_vs_fp64_max": max(rel_err_per_rhs(xb, x64, nrhs))}
+        del xb
 
     # residual of the Dirac equation M_pc x = xi itself (not of the normal
     # equations the CG works on) for rhs 0, untimed: the same solver on the
@@ -344,7 +449,7 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(cfg, u_w, src, gpu_fp64_iters)
+        cpu = cpu_baseline(cfg, u_w, src, gpu_fp64_iters, full=a.cpu_full)
 
     line = {
         "metric": "DWF CG time-to-solution/RHS @1e-10 & dslash HBM GB/s: mixed+batched vs FP64",
@@ -372,6 +477,8 @@ def main():
         "reliable_updates": [r.reliable_updates for r in res_main],
         "fp64": fp64,
         "speedup_vs_fp64": (fp64["s_per_rhs"] / s_per_rhs) if fp64 else None,
+        "err_vs_fp64": err64,
+        "bf16_links": bf16,
         "dslash": {"kernel": "k_hop (5-D DWF)" if cfg["kind"] == C.OP_DWF else "k_hop (4-D, batched)",
                    "seconds_per_launch": sec, "algorithmic_bytes": byt, "GBps": achieved,
                    "fp64_GBps": byt64 / sec64 / 1e9, "fp64_seconds_per_launch": sec64},

@@ -56,6 +56,17 @@ class NerscInfo(ctypes.Structure):
                 ("link_trace_header", ctypes.c_double), ("link_trace_data", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    """cps_hip_tuning: process-wide solver and launch controls (an environment
+    variable CPS_HIP_<NAME>, when set, overrides the field)."""
+    _fields_ = [("use_graph", ctypes.c_int), ("tune", ctypes.c_int), ("tune_report", ctypes.c_int),
+                ("hop_mode", ctypes.c_int), ("rhs_window", ctypes.c_int), ("lagged_x", ctypes.c_int),
+                ("paired_x", ctypes.c_int), ("s5_xpay", ctypes.c_int), ("r_in_hop", ctypes.c_int),
+                ("clover_r_in_hop", ctypes.c_int), ("m5_fused", ctypes.c_int), ("s5_fused", ctypes.c_int),
+                ("s5x_row", ctypes.c_int), ("sloppy_vectors", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("progress_seconds", ctypes.c_double)]
+
+
 class CgResult(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("status", ctypes.c_int), ("true_res", ctypes.c_double),
                 ("src_norm_sq", ctypes.c_double), ("reliable_updates", ctypes.c_int), ("seconds", ctypes.c_double)]
@@ -114,6 +125,9 @@ def _declare(l):
         "cps_hip_service_destroy": [vp],
         "cps_hip_service_add_gauge": [vp, vp, i, vp, vp, P(i)],
         "cps_hip_service_submit": [vp, i, P(Op), vp, vp, i, d, i, P(ctypes.c_longlong)],
+        "cps_hip_service_submit_arg": [vp, i, P(Op), vp, vp, i, P(CgArg), P(ctypes.c_longlong)],
+        "cps_hip_get_tuning": [P(Tuning)],
+        "cps_hip_set_tuning": [P(Tuning)],
         "cps_hip_service_wait": [vp, ctypes.c_longlong, P(CgResult), P(d)],
         "cps_hip_service_get_stats": [vp, P(ServiceStats)],
         "cps_hip_device_free": [vp],
@@ -136,6 +150,16 @@ def _check(rc, what):
 def _ptr(a):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def get_tuning():
+    t = Tuning()
+    _check(lib().cps_hip_get_tuning(ctypes.byref(t)), "cps_hip_get_tuning")
+    return t
+
+
+def set_tuning(t):
+    _check(lib().cps_hip_set_tuning(ctypes.byref(t)), "cps_hip_set_tuning")
 
 
 def make_op(kind, mass=0.0, m5=1.8, a5_inv=1.0, b=1.0, c=0.0, csw=0.0):
@@ -375,13 +399,22 @@ class Service:
         self._gauge_ls[gid.value] = (list(node_sites), int(ls))
         return gid.value
 
-    def submit(self, gauge_id, op, src, nrhs=1, stop_rsd=1e-10, max_num_iter=10000):
+    def submit(self, gauge_id, op, src, nrhs=1, stop_rsd=1e-10, max_num_iter=10000, precision=PREC_DOUBLE,
+               reliable_every=100, reliable_delta=0.1):
+        """precision PREC_DOUBLE: FP64 CG; a mixed mode: the sloppy CG of that
+        mode with FP64 reliable updates (cps_hip_service_submit_arg)."""
         src = np.ascontiguousarray(src, dtype=np.float64)
         sol = np.zeros_like(src)
         rid = ctypes.c_longlong()
-        _check(lib().cps_hip_service_submit(self.h, int(gauge_id), ctypes.byref(op), _ptr(src), _ptr(sol), int(nrhs),
-                                            float(stop_rsd), int(max_num_iter), ctypes.byref(rid)),
-               "cps_hip_service_submit")
+        if precision == PREC_DOUBLE:
+            _check(lib().cps_hip_service_submit(self.h, int(gauge_id), ctypes.byref(op), _ptr(src), _ptr(sol),
+                                                int(nrhs), float(stop_rsd), int(max_num_iter), ctypes.byref(rid)),
+                   "cps_hip_service_submit")
+        else:
+            arg = make_cg_arg(stop_rsd, max_num_iter, precision, reliable_every, reliable_delta)
+            _check(lib().cps_hip_service_submit_arg(self.h, int(gauge_id), ctypes.byref(op), _ptr(src), _ptr(sol),
+                                                    int(nrhs), ctypes.byref(arg), ctypes.byref(rid)),
+                   "cps_hip_service_submit_arg")
         self._live[rid.value] = (src, sol, nrhs)   # buffers stay alive until wait()
         return rid.value
 

@@ -46,8 +46,13 @@ __global__ void k_gauge_canon_to_wilson(const double* __restrict__ canon, double
 #pragma unroll
     for (int col = 0; col < 3; col++) {
       const double a = src[2 * (3 * row + col)], b = src[2 * (3 * row + col) + 1];
-      dst[2 * (row + 3 * col)] = fr * a - fi * b;
-      dst[2 * (row + 3 * col) + 1] = fr * b + fi * a;
+      // products rounded separately (no fused multiply-add), as a host
+      // BondCond without contraction computes the complex product
+      {
+#pragma clang fp contract(off)
+        dst[2 * (row + 3 * col)] = fr * a - fi * b;
+        dst[2 * (row + 3 * col) + 1] = fr * b + fi * a;
+      }
     }
 }
 

@@ -114,14 +114,22 @@ struct HopParams {
   int rhs_per_set;
   int rhs0;          // clover set of local rhs r: (rhs0 + r) / rhs_per_set
   CloverRef clover;
+  // CLOV_X: kscale[set] as above; CLOV_KS (a mass per rhs, Wilson kind):
+  // the hop's scale of local rhs r is scale * kscale[rhs0 + r]
   const double* kscale;
+  // CLOV_KS, DWF kind: the wall coefficient (-2 mf / a5) of local rhs r is
+  // kwall[rhs0 + r] instead of neg_mass_two_over_a5
+  const double* kwall;
 };
 
 // CLOV_S5XR: CLOV_S5X with the row exchange of s5_row; CLOV_S5XRR: the same
 // with the CG residual update (rupd) instead of storing out; CLOV_XRR:
 // CLOV_X with that residual update (kernels of their own, chosen by the
 // launcher; not values of HopParams::clov)
-enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2, CLOV_S5X = 3, CLOV_S5XR = 4, CLOV_S5XRR = 5, CLOV_XRR = 6 };
+// CLOV_KS: no clover term, a mass per rhs (kscale for the Wilson kind, kwall
+// for the DWF kind), element order only; a value of HopParams::clov
+enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2, CLOV_S5X = 3, CLOV_S5XR = 4, CLOV_S5XRR = 5, CLOV_XRR = 6,
+       CLOV_KS = 7 };
 
 hipError_t launch_hop(const HopParams& P, hipStream_t st);
 hipError_t build_nbr(const Geom& g, uint4* nbr, uint32_t* nbr_neg, hipStream_t st);

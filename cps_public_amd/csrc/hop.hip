@@ -34,6 +34,19 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
   // the fused fifth-dimension stage needs the 16 slots of a site in one row
   if (P.m5post && (!std::is_same<SIN, SpinH>::value || P.kind != KIND_WILSON || P.nb != 16 || P.X.data || bs % 16))
     return hipErrorInvalidValue;
+  if (P.clov == CLOV_KS) {
+    // a mass per rhs: element order, no variants
+    if (P.flags & (HOP_SPATIAL | HOP_SITEWAVE | HOP_LDS_LINKS | HOP_RHSGROUP) || P.m5post) return hipErrorInvalidValue;
+    if (P.kind == KIND_DWF ? !P.kwall : !P.kscale) return hipErrorInvalidValue;
+#define CPSHIP_HOP_KS(DAG, KIND) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_KS, 0>), grid, block, 0, st, Q);
+    if (P.kind == KIND_DWF) {
+      if (P.dag) { CPSHIP_HOP_KS(1, KIND_DWF) } else { CPSHIP_HOP_KS(0, KIND_DWF) }
+    } else {
+      if (P.dag) { CPSHIP_HOP_KS(1, KIND_WILSON) } else { CPSHIP_HOP_KS(0, KIND_WILSON) }
+    }
+#undef CPSHIP_HOP_KS
+    return hipGetLastError();
+  }
 #define CPSHIP_HOP_LAUNCH(DAG, KIND, SW)                                                                      \
   if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, true, CLOV_NONE, SW>), grid, block, shm, st, Q); \
   else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_NONE, SW>), grid, block, shm, st, Q);
@@ -60,7 +73,7 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
 }
 
 hipError_t launch_hop(const HopParams& P, hipStream_t st) {
-  if (P.clov != CLOV_NONE) return launch_hop_clover(P, st);
+  if (P.clov != CLOV_NONE && P.clov != CLOV_KS) return launch_hop_clover(P, st);
   switch (P.prec) {
     case PREC_D_D18: return launch_t<SpinD, SpinD, LinkD18, double>(P, st);
     case PREC_F_F12: return launch_t<SpinF, SpinF, LinkF12, float>(P, st);

@@ -885,8 +885,12 @@ __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem&
       // in slots at this site hold s' = 2j' + (1 - cls_out)
       const int sm = (s == 0) ? ls - 1 : s - 1;
       const int sp = (s == ls - 1) ? 0 : s + 1;
-      const R cm = (s == 0) ? R(P.neg_mass_two_over_a5) : R(P.two_over_a5);
-      const R cp = (s == ls - 1) ? R(P.neg_mass_two_over_a5) : R(P.two_over_a5);
+      // wall coefficient -2 mf / a5: one for the launch, or per rhs (CLOV_KS:
+      // kwall, a kernel of its own so that no conditional load enters the
+      // stencil)
+      const R cw = CLOV == CLOV_KS ? R(P.kwall[P.rhs0 + rhs]) : R(P.neg_mass_two_over_a5);
+      const R cm = (s == 0) ? cw : R(P.two_over_a5);
+      const R cp = (s == ls - 1) ? cw : R(P.two_over_a5);
       const int jm = sm >> 1, jp = sp >> 1;
       // dag 0: upper from s-1, lower from s+1; dag 1: lower from s-1, upper from s+1
       const int c_from_m = (DAG == 0) ? 0 : 6;
@@ -968,7 +972,7 @@ __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem&
           res[c_from_p + k].im += scp * pim;
         }
       }
-    } else if constexpr (CLOV != CLOV_NONE) {
+    } else if constexpr (CLOV != CLOV_NONE && CLOV != CLOV_KS) {
       const int set = (P.rhs0 + rhs) / P.rhs_per_set;
       // (no branch here: a conditional load splits the block and the stencil
       // above it then spills)
@@ -990,7 +994,11 @@ __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem&
           res[k] = cmk<R>(xc * ax[k].re + sc * acc[k].re, xc * ax[k].im + sc * acc[k].im);
       }
     } else {
-      const R sc = R(P.scale);
+      // scale of the hop: one for the launch, or (CLOV_KS, Wilson kind)
+      // times kscale[rhs0 + rhs], the per-rhs kappa^2 of the Wilson M
+      R sc;
+      if constexpr (CLOV == CLOV_KS && KIND == KIND_WILSON) sc = R(P.scale * P.kscale[P.rhs0 + rhs]);
+      else sc = R(P.scale);
       if (P.X.data) {
         cplx<R> xv[12];
         spin_load<SOUT, R>(P.X, eo, xv);

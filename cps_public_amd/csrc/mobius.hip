@@ -529,16 +529,10 @@ struct RUpdate {
 };
 
 // CPS_HIP_M5_FUSED=0 keeps M5^-1 as its own pass
-static bool m5_fused_enabled() {
-  const char* e = getenv("CPS_HIP_M5_FUSED");
-  return !(e && *e == '0');
-}
+static bool m5_fused_enabled() { return knob(&cps_hip_tuning::m5_fused, "CPS_HIP_M5_FUSED") != 0; }
 
 // CPS_HIP_S5_FUSED=0 keeps the dag-1 s-term as its own pass
-static bool s5_fused_enabled() {
-  const char* e = getenv("CPS_HIP_S5_FUSED");
-  return !(e && *e == '0');
-}
+static bool s5_fused_enabled() { return knob(&cps_hip_tuning::s5_fused, "CPS_HIP_S5_FUSED") != 0; }
 
 // s5x: out = X + x5coef (P X) + scale (hb5 hop + hc5 P hop) (CLOV_S5X)
 static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag, double scale,
@@ -574,8 +568,7 @@ static int hop4(Lattice& L, int hprec, const FieldRef& out, const FieldRef& in, 
     P.hb5 = hb5;
     P.hc5 = hc5;
     // CPS_HIP_S5X_ROW=0 loads the slots s-1, s+1 of X instead
-    const char* e = getenv("CPS_HIP_S5X_ROW");
-    P.s5_row = (store_of(hprec) == STORE_H && L.ls == 16 && !(e && *e == '0')) ? 1 : 0;
+    P.s5_row = (store_of(hprec) == STORE_H && L.ls == 16 && knob(&cps_hip_tuning::s5x_row, "CPS_HIP_S5X_ROW") != 0) ? 1 : 0;
   }
   if (ru) {
     P.s5_row = 1;   // the residual step exists in the row variant only

@@ -2,15 +2,17 @@
 // propagator solves (any registered lattice, operator and mass, 1..n
 // right-hand sides) are served by continuous batching.
 //
-// An engine is one batched FP64 CG (the iteration of DiracOp::InvCg,
-// inv_cg.C:293-364) over S right-hand-side slots for one (lattice, operator)
-// pair, on its own HIP stream.  Every slot carries its own CG scalars, so a
-// slot that converges is harvested (FP64 true residual, solution copied out)
-// and refilled with the next waiting right-hand side while the other slots
-// keep iterating: requests that arrive while others are on the GPU join as
-// slots free up.  Engines of different lattices or masses run concurrently
-// on their own streams; a new engine is created only while the HBM budget
-// allows (admission by footprint), and idle engines are released.
+// An engine is one batched CG (FP64: the iteration of DiracOp::InvCg,
+// inv_cg.C:293-364; mixed: the sloppy iteration with FP64 reliable updates)
+// over S right-hand-side slots for one lattice, operator and precision, on
+// its own HIP stream; Wilson and DWF engines carry a mass per slot.  Every
+// slot carries its own CG scalars, so a slot that converges is harvested
+// (FP64 true residual, solution copied out) and refilled with the next
+// waiting right-hand side while the other slots keep iterating: requests
+// that arrive while others are on the GPU join as slots free up.  Engines of
+// different lattices or operators run concurrently on their own streams; a
+// new engine is created only while the HBM budget allows (admission by
+// footprint), and idle engines are released.
 //
 // A right-hand side's arithmetic never depends on the other slots: the hop
 // and BLAS kernels are element-wise, and every reduction sums one slot's
@@ -44,6 +46,9 @@ struct Request {
   int nrhs = 0;
   double stop_rsd = 1e-10;
   int max_iter = 10000;
+  int precision = CPS_HIP_PREC_DOUBLE;
+  int reliable_every = 100;
+  double reliable_delta = 0.1;
   int next_rhs = 0;              // next rhs to place in a slot
   int done_rhs = 0;
   int rc = 0;
@@ -59,29 +64,37 @@ struct GaugeEntry {
   DevBuf u;   // WILSON order FP64, resident
 };
 
-bool same_op(const cps_hip_op& a, const cps_hip_op& b) {
-  return a.kind == b.kind && a.mass == b.mass && a.m5 == b.m5 && a.a5_inv == b.a5_inv && a.mobius_b == b.mobius_b &&
-         a.mobius_c == b.mobius_c && a.csw == b.csw;
+// Wilson and DWF engines take a mass per slot (the mass enters only the
+// second hop's kappa^2, or the DWF wall coefficient), so requests of any
+// mass on one lattice share an engine; Moebius and clover engines are per
+// mass (the mass enters M5^-1 and the clover term).
+bool per_slot_mass(const cps_hip_op& op) { return op.kind == CPS_HIP_OP_WILSON || op.kind == CPS_HIP_OP_DWF; }
+
+bool same_engine_op(const cps_hip_op& a, const cps_hip_op& b) {
+  return a.kind == b.kind && (per_slot_mass(a) || a.mass == b.mass) && a.m5 == b.m5 && a.a5_inv == b.a5_inv &&
+         a.mobius_b == b.mobius_b && a.mobius_c == b.mobius_c && a.csw == b.csw;
 }
 
-FieldRef slot_ref(const Field& f, int64_t per_rhs, int k) {
-  FieldRef r = f.ref();
-  r.data = static_cast<char*>(r.data) + (size_t)k * per_rhs * 16;   // FP64: 16-byte chunks per plane
-  return r;
-}
-
-CGScalars slot_scalars(const CGScalars& s, int k) {
-  CGScalars v = s;
-  v.rr += k; v.rr_prev += k; v.d += k; v.alpha += k; v.beta += k; v.stp += k; v.src_norm += k; v.rsd += k;
-  v.active += k; v.iters += k; v.status += k;
-  return v;
-}
-
+// An engine: one batched CG over S right-hand-side slots for one lattice,
+// operator and solver precision, on the lattice's own stream.
+//  - FP64: the iteration of DiracOp::InvCg (inv_cg.C:293-364) per slot; a
+//    slot that stopped gets its FP64 true residual (inv_cg.C:403-417) alone.
+//  - mixed (FP32 arithmetic on the sloppy fields of the precision mode):
+//    the sloppy CG per slot with FP64 reliable updates of all slots at
+//    least every `every` iterations and whenever a slot's sloppy residual
+//    stops it (below delta times its last FP64 residual, or the stopping
+//    rule): x += xs and r = b - M^dag M x in FP64, the sloppy CG continuing
+//    from that r; a slot finishes only on its FP64 residual.
 struct Engine {
   int gauge_id = -1;
   cps_hip_op op;
   int max_iter = 0;
+  int prec = CPS_HIP_PREC_DOUBLE;
+  int every = 100;
+  double delta = 0.1;
   int S = 0;
+  bool mixed = false, slot_mass = false;
+  int hprec = PREC_D_D18, sps = STORE_D, svec = STORE_D, xvec = STORE_D;
   Lattice L;
   OpLayout o;
   int bsb = 0, bpr_blas = 0, bpr_hop = 0;
@@ -89,24 +102,47 @@ struct Engine {
   CGState cg;
   Field *x = nullptr, *b = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *t1 = nullptr, *t2 = nullptr,
         *t3 = nullptr;
+  Field *xs = nullptr, *rs = nullptr, *ps = nullptr, *aps = nullptr, *t1s = nullptr, *t2s = nullptr;
+  DevBuf mass_buf;   // per slot: kappa^2 [0, S), wall coefficient [S, 2S)
+  DevBuf coef_buf;
   std::vector<Request*> owner;
-  std::vector<int> rhs_of;
+  std::vector<int> rhs_of, updates;
+  int since = 0;     // mixed: iterations since the last reliable update
   long long bytes = 0;
   clk::time_point idle_since = clk::now();
   std::unique_ptr<IterGraph> graph;
 
-  static long long footprint(const GaugeEntry& ge, const cps_hip_op& op, int S) {
+  static bool is_mixed(int prec) { return prec != CPS_HIP_PREC_DOUBLE; }
+
+  static long long footprint(const GaugeEntry& ge, const cps_hip_op& op, int S, int prec) {
     const long long V = (long long)ge.L[0] * ge.L[1] * ge.L[2] * ge.L[3];
     const long long per_rhs = (op.kind == CPS_HIP_OP_DWF || op.kind == CPS_HIP_OP_MOBIUS) ? V / 2 * ge.ls : V / 2;
     const long long vec = per_rhs * 192;
-    return 8 * vec * S + vec + 2 * V * 72 * 8 + V * 48 + 64 * 1024 * 1024;
+    long long b = 8 * vec * S + vec + 2 * V * 72 * 8 + V * 48 + 64 * 1024 * 1024;
+    if (is_mixed(prec)) b += S * per_rhs * (96 + 5 * 96) + 2 * V * 72 * 4;   // sloppy fields (at most FP32), links
+    return b;
   }
 
-  int setup(const GaugeEntry& ge, int gid, const cps_hip_op& op_, int S_, int max_iter_) {
+  const double* k2() const { return slot_mass ? (const double*)mass_buf.p : nullptr; }
+  const double* wall() const { return slot_mass ? (const double*)mass_buf.p + S : nullptr; }
+
+  // M^dag M on slots [lo, lo + n) of the engine's fields
+  int mdagm(int hp, const FieldRef& out, const FieldRef& in, const FieldRef& a, const FieldRef& c, double* pt, int n,
+            int lo) {
+    if (slot_mass) return apply_mdagm_slots(L, &op, hp, out, in, a, c, pt, n, k2(), wall(), lo);
+    return apply_mdagm(L, &op, hp, out, in, a, c, pt, n);
+  }
+
+  int setup(const GaugeEntry& ge, int gid, const Request& rq, int S_) {
     gauge_id = gid;
-    op = op_;
+    op = rq.op;
     S = S_;
-    max_iter = max_iter_;
+    max_iter = rq.max_iter;
+    prec = rq.precision;
+    every = rq.reliable_every > 0 ? rq.reliable_every : 100;
+    delta = rq.reliable_delta > 0 ? rq.reliable_delta : 0.1;
+    mixed = is_mixed(prec);
+    slot_mass = per_slot_mass(op);
     int rc = L.init(ge.L, ge.ls, ge.bc);
     if (rc) return rc;
     if ((rc = L.import_gauge_device((const double*)ge.u.p))) return rc;
@@ -124,6 +160,19 @@ struct Engine {
     t1 = &L.field(S_T1, STORE_D, n);
     t2 = &L.field(S_T2, STORE_D, n);
     t3 = &L.field(S_T3, STORE_D, n);
+    if (mixed) {
+      // the batch solver's formats for this mode with S rhs; the sloppy
+      // residual in the direction's storage, the sloppy solution FP32
+      precision_formats(prec, &op, L.ls, S, L.recon_ok, &sps, &hprec);
+      svec = sps;
+      xvec = STORE_F;
+      xs = &L.field(S_XS, xvec, n);
+      rs = &L.field(S_RS, svec, n);
+      ps = &L.field(S_PS, sps, n);
+      aps = &L.field(S_APS, sps, n);
+      t1s = &L.field(S_T1S, sps, n);
+      t2s = &L.field(S_T2S, sps, n);
+    }
     if (L.last_rc) return L.last_rc;
     bsb = pick_block(o.per_rhs);
     bpr_blas = (int)(o.per_rhs / bsb);
@@ -132,14 +181,31 @@ struct Engine {
     part = (double*)L.partial.p;
     if ((rc = cg.alloc(L, S, 0))) return rc;
     cg.s.max_itr = max_iter - 1;
+    if (mixed) {
+      cg.s.rr_upd = cg.upd_store;
+      cg.s.delta2 = delta * delta;
+    }
+    if ((rc = mass_buf.ensure(2 * S * sizeof(double)))) return rc;
+    if ((rc = coef_buf.ensure(S * sizeof(double)))) return rc;
     if ((rc = L.io.ensure((size_t)o.per_rhs * 24 * sizeof(double)))) return rc;
-    // empty slots hold zeros and are inactive
+    // empty slots hold zeros and are inactive (and converged, so that a
+    // reliable update leaves them alone)
     for (Field* f : {x, b, r, p, ap, t1, t2, t3}) CPSHIP_CHECK(blas_zero(STORE_D, f->ref(), o.per_rhs, S, bsb, L.stream));
-    std::vector<int> zeros(S, 0);
+    if (mixed)
+      for (Field* f : {xs, rs, ps, aps, t1s, t2s})
+        CPSHIP_CHECK(blas_zero(f->storage, f->ref(), o.per_rhs, S, bsb, L.stream));
+    std::vector<int> zeros(S, 0), conv(S, CG_STATUS_CONVERGED);
+    std::vector<double> dz(2 * S, 0.0);
     CPSHIP_CHECK(hipMemcpyAsync(cg.s.active, zeros.data(), S * sizeof(int), hipMemcpyHostToDevice, L.stream));
+    CPSHIP_CHECK(hipMemcpyAsync(cg.s.iters, zeros.data(), S * sizeof(int), hipMemcpyHostToDevice, L.stream));
+    CPSHIP_CHECK(hipMemcpyAsync(cg.s.status, conv.data(), S * sizeof(int), hipMemcpyHostToDevice, L.stream));
+    for (double* d : {cg.s.stp, cg.s.src_norm, cg.s.alpha, cg.s.beta, cg.s.rr, cg.s.rr_prev})
+      CPSHIP_CHECK(hipMemcpyAsync(d, dz.data(), S * sizeof(double), hipMemcpyHostToDevice, L.stream));
+    CPSHIP_CHECK(hipMemcpyAsync(mass_buf.p, dz.data(), 2 * S * sizeof(double), hipMemcpyHostToDevice, L.stream));
     CPSHIP_CHECK(hipStreamSynchronize(L.stream));
     owner.assign(S, nullptr);
     rhs_of.assign(S, -1);
+    updates.assign(S, 0);
     bytes = L.device_bytes();
     graph.reset(new IterGraph(true));
     return 0;
@@ -153,42 +219,104 @@ struct Engine {
   bool busy() const { return free_slots() < S; }
 
   // rhs k of request rq into slot s: x = 0, r = p = src, |src|^2, stp, |r|^2
+  // (mixed: the sloppy residual = src, xs = 0, p = 0 with beta = 0, and the
+  // slot's last FP64 residual = |src|^2); the slot's mass
   int admit(int s, Request* rq, int k) {
     const double* h = rq->src + (size_t)k * o.per_rhs * 24;
     CPSHIP_CHECK(hipMemcpyAsync(L.io.p, h, (size_t)o.per_rhs * 24 * sizeof(double), hipMemcpyHostToDevice, L.stream));
-    const FieldRef bs = slot_ref(*b, o.per_rhs, s), xs = slot_ref(*x, o.per_rhs, s);
-    const FieldRef rs = slot_ref(*r, o.per_rhs, s), ps = slot_ref(*p, o.per_rhs, s);
+    const FieldRef bs = rhs_view(b->ref(), o.per_rhs, s), xs0 = rhs_view(x->ref(), o.per_rhs, s);
+    const FieldRef rs0 = rhs_view(r->ref(), o.per_rhs, s), ps0 = rhs_view(p->ref(), o.per_rhs, s);
     CPSHIP_CHECK(pack_spinor(STORE_D, (const double*)L.io.p, bs, o.map, L.g.Vcb, L.ls, 1, o.per_rhs, L.stream));
-    CPSHIP_CHECK(blas_zero(STORE_D, xs, o.per_rhs, 1, bsb, L.stream));
-    CPSHIP_CHECK(convert_spinor(STORE_D, bs, STORE_D, rs, o.per_rhs, L.stream));
-    CPSHIP_CHECK(convert_spinor(STORE_D, bs, STORE_D, ps, o.per_rhs, L.stream));
+    CPSHIP_CHECK(blas_zero(STORE_D, xs0, o.per_rhs, 1, bsb, L.stream));
+    CPSHIP_CHECK(convert_spinor(STORE_D, bs, STORE_D, rs0, o.per_rhs, L.stream));
+    CPSHIP_CHECK(convert_spinor(STORE_D, bs, STORE_D, ps0, o.per_rhs, L.stream));
+    if (slot_mass) {
+      const double kap = 1.0 / (2.0 * (rq->op.mass + 4.0));
+      const double k2v = kap * kap, wv = -2.0 * rq->op.mass * op.a5_inv;
+      CPSHIP_CHECK(hipMemcpyAsync((double*)mass_buf.p + s, &k2v, sizeof(double), hipMemcpyHostToDevice, L.stream));
+      CPSHIP_CHECK(hipMemcpyAsync((double*)mass_buf.p + S + s, &wv, sizeof(double), hipMemcpyHostToDevice, L.stream));
+    }
     CPSHIP_CHECK(hipMemcpyAsync(cg.s.rsd + s, &rq->stop_rsd, sizeof(double), hipMemcpyHostToDevice, L.stream));
-    const CGScalars v = slot_scalars(cg.s, s);
+    const CGScalars v = scalars_view(cg.s, s);
     CPSHIP_CHECK(blas_norm(STORE_D, bs, o.per_rhs, 1, bsb, part, L.stream));
     CPSHIP_CHECK(reduce_rhs(part, bpr_blas, 1, nullptr, RED_SRC_NORM, v, L.stream));
-    CPSHIP_CHECK(blas_norm(STORE_D, rs, o.per_rhs, 1, bsb, part, L.stream));
-    CPSHIP_CHECK(reduce_rhs(part, bpr_blas, 1, nullptr, RED_CG_INIT, v, L.stream));
+    CPSHIP_CHECK(blas_norm(STORE_D, rs0, o.per_rhs, 1, bsb, part, L.stream));
+    CPSHIP_CHECK(reduce_rhs(part, bpr_blas, 1, mixed ? v.rr_upd : nullptr, RED_CG_INIT, v, L.stream));
+    if (mixed) {
+      CPSHIP_CHECK(convert_spinor(STORE_D, bs, svec, rhs_view(rs->ref(), o.per_rhs, s), o.per_rhs, L.stream));
+      CPSHIP_CHECK(blas_zero(xvec, rhs_view(xs->ref(), o.per_rhs, s), o.per_rhs, 1, bsb, L.stream));
+      CPSHIP_CHECK(blas_zero(sps, rhs_view(ps->ref(), o.per_rhs, s), o.per_rhs, 1, bsb, L.stream));
+      const double z = 0.0;
+      CPSHIP_CHECK(hipMemcpyAsync(cg.s.beta + s, &z, sizeof(double), hipMemcpyHostToDevice, L.stream));
+      CPSHIP_CHECK(hipMemcpyAsync(cg.s.alpha + s, &z, sizeof(double), hipMemcpyHostToDevice, L.stream));
+    }
     owner[s] = rq;
     rhs_of[s] = k;
+    updates[s] = 0;
     return 0;
   }
 
-  // `iters` CG iterations over all slots (inactive slots are frozen by the
-  // per-slot scalars), enqueued on the engine's stream
-  int step(int iters) {
-    return graph->run(L, iters, [&]() -> int {
-      int e = apply_mdagm(L, &op, PREC_D_D18, ap->ref(), p->ref(), t1->ref(), t2->ref(), part, S);
+  // up to `iters` CG iterations over all slots (inactive slots are frozen by
+  // the per-slot scalars), enqueued on the engine's stream; returns the
+  // number run (mixed: up to the next reliable update)
+  int step(int iters, int* ran) {
+    if (!mixed) {
+      *ran = iters;
+      return graph->run(L, iters, [&]() -> int {
+        int e = mdagm(PREC_D_D18, ap->ref(), p->ref(), t1->ref(), t2->ref(), part, S, 0);
+        if (e) return e;
+        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, S, nullptr, RED_CG_ALPHA, cg.s, L.stream));
+        CPSHIP_CHECK(blas_cg_xr_d(x->ref(), p->ref(), r->ref(), ap->ref(), cg.s, o.per_rhs, S, bsb, part, L.stream));
+        CPSHIP_CHECK(reduce_rhs(part, bpr_blas, S, nullptr, RED_CG_BETA, cg.s, L.stream));
+        CPSHIP_CHECK(blas_xpay(STORE_D, p->ref(), r->ref(), cg.s.beta, cg.s.active, o.per_rhs, S, bsb, L.stream));
+        return 0;
+      });
+    }
+    const int n = std::max(1, std::min(iters, every - since));
+    *ran = n;
+    since += n;
+    // one sloppy iteration: p = r + beta p, A p, alpha, xs += alpha p and
+    // r -= alpha A p with |r|^2, beta (the batch solver's unfused order)
+    return graph->run(L, n, [&]() -> int {
+      CPSHIP_CHECK(blas_xpay_mixed(sps, ps->ref(), svec, rs->ref(), cg.s.beta, cg.s.active, o.per_rhs, S, bsb,
+                                   L.stream));
+      int e = mdagm(hprec, aps->ref(), ps->ref(), t1s->ref(), t2s->ref(), part, S, 0);
       if (e) return e;
       CPSHIP_CHECK(reduce_rhs(part, bpr_hop, S, nullptr, RED_CG_ALPHA, cg.s, L.stream));
-      CPSHIP_CHECK(blas_cg_xr_d(x->ref(), p->ref(), r->ref(), ap->ref(), cg.s, o.per_rhs, S, bsb, part, L.stream));
+      CPSHIP_CHECK(blas_cg_xr_mixed(xvec, xs->ref(), sps, ps->ref(), svec, rs->ref(), sps, aps->ref(), cg.s, o.per_rhs,
+                                    S, bsb, part, L.stream));
       CPSHIP_CHECK(reduce_rhs(part, bpr_blas, S, nullptr, RED_CG_BETA, cg.s, L.stream));
-      CPSHIP_CHECK(blas_xpay(STORE_D, p->ref(), r->ref(), cg.s.beta, cg.s.active, o.per_rhs, S, bsb, L.stream));
       return 0;
     });
   }
 
+  // the FP64-critical steps for every slot: x += xs, r = b - M^dag M x,
+  // |r|^2 into rr_true, the sloppy residual restarted from r, p
+  // re-orthogonalised; slots whose FP64 residual meets the stopping rule
+  // finish (mixed_after_update)
+  int reliable_update() {
+    CPSHIP_CHECK(blas_axpy(xvec, xs->ref(), STORE_D, x->ref(), nullptr, 1.0, o.per_rhs, S, bsb, L.stream));
+    CPSHIP_CHECK(blas_zero(xvec, xs->ref(), o.per_rhs, S, bsb, L.stream));
+    int e = mdagm(PREC_D_D18, ap->ref(), x->ref(), t1->ref(), t2->ref(), nullptr, S, 0);
+    if (e) return e;
+    CPSHIP_CHECK(blas_sub_norm_d(b->ref(), ap->ref(), r->ref(), o.per_rhs, S, bsb, part, L.stream));
+    CPSHIP_CHECK(reduce_rhs(part, bpr_blas, S, cg.rr_true, RED_PLAIN, cg.s, L.stream));
+    CPSHIP_CHECK(convert_spinor(STORE_D, r->ref(), svec, rs->ref(), o.per_rhs * S, L.stream));
+    CPSHIP_CHECK(blas_redot_mixed(svec, rs->ref(), sps, ps->ref(), o.per_rhs, S, bsb, part, L.stream));
+    CPSHIP_CHECK(reduce_rhs(part, bpr_blas, S, cg.s.alpha, RED_PLAIN, cg.s, L.stream));
+    double* coef = (double*)coef_buf.p;
+    CPSHIP_CHECK(mixed_after_update(cg.s, cg.rr_true, cg.s.alpha, coef, S, L.stream));
+    CPSHIP_CHECK(blas_axpy(svec, rs->ref(), sps, ps->ref(), coef, 0.0, o.per_rhs, S, bsb, L.stream));
+    for (int s = 0; s < S; s++)
+      if (owner[s]) updates[s]++;
+    since = 0;
+    return 0;
+  }
+
   // Slots whose CG stopped: FP64 true residual (inv_cg.C:403-417), solution
-  // copied to the caller, slot freed.  Returns the requests touched.
+  // copied to the caller, slot freed.  One synchronisation reads the slots'
+  // flags; the residuals, solutions and counters of every finished slot are
+  // then enqueued together and read back with one more.
   int harvest(std::vector<Request*>& touched) {
     std::vector<int> act(S), st(S);
     CPSHIP_CHECK(hipMemcpyAsync(act.data(), cg.s.active, S * sizeof(int), hipMemcpyDeviceToHost, L.stream));
@@ -197,33 +325,57 @@ struct Engine {
     bool any = false;
     for (int s = 0; s < S; s++)
       if (owner[s] && !act[s]) any = true;
-    if (!any) return 0;
-    int e = apply_mdagm(L, &op, PREC_D_D18, ap->ref(), x->ref(), t1->ref(), t2->ref(), nullptr, S);
-    if (e) return e;
-    CPSHIP_CHECK(blas_sub_norm_d(b->ref(), ap->ref(), t3->ref(), o.per_rhs, S, bsb, part, L.stream));
-    double* trn = cg.s.d;   // free between iterations (the next one rewrites it first)
-    CPSHIP_CHECK(reduce_rhs(part, bpr_blas, S, trn, RED_PLAIN, cg.s, L.stream));
+    if (!any && !(mixed && since >= every)) return 0;
+    double* trn = cg.rr_true;
+    if (mixed) {
+      // a reliable update decides which stopped slots are finished (FP64
+      // residual) and restarts the others
+      if (int e = reliable_update()) return e;
+      CPSHIP_CHECK(hipMemcpyAsync(act.data(), cg.s.active, S * sizeof(int), hipMemcpyDeviceToHost, L.stream));
+      CPSHIP_CHECK(hipMemcpyAsync(st.data(), cg.s.status, S * sizeof(int), hipMemcpyDeviceToHost, L.stream));
+      CPSHIP_CHECK(hipStreamSynchronize(L.stream));
+    }
+    std::vector<int> done;
+    for (int s = 0; s < S; s++)
+      if (owner[s] && !act[s] && st[s] != CG_STATUS_NEED_UPDATE && st[s] != CG_STATUS_RUNNING) done.push_back(s);
+    if (done.empty()) return 0;
+    if (!mixed) {
+      // the FP64 true residual of the stopped slots only
+      for (int s : done) {
+        const FieldRef xw = rhs_view(x->ref(), o.per_rhs, s), aw = rhs_view(ap->ref(), o.per_rhs, s);
+        int e = mdagm(PREC_D_D18, aw, xw, t1->ref(), t2->ref(), nullptr, 1, s);
+        if (e) return e;
+        CPSHIP_CHECK(blas_sub_norm_d(rhs_view(b->ref(), o.per_rhs, s), aw, rhs_view(t3->ref(), o.per_rhs, s), o.per_rhs,
+                                     1, bsb, part, L.stream));
+        CPSHIP_CHECK(reduce_rhs(part, bpr_blas, 1, trn + s, RED_PLAIN, scalars_view(cg.s, s), L.stream));
+      }
+    }
     std::vector<double> tr(S), sn(S);
     std::vector<int> it(S);
     CPSHIP_CHECK(hipMemcpyAsync(tr.data(), trn, S * sizeof(double), hipMemcpyDeviceToHost, L.stream));
     CPSHIP_CHECK(hipMemcpyAsync(sn.data(), cg.s.src_norm, S * sizeof(double), hipMemcpyDeviceToHost, L.stream));
     CPSHIP_CHECK(hipMemcpyAsync(it.data(), cg.s.iters, S * sizeof(int), hipMemcpyDeviceToHost, L.stream));
-    CPSHIP_CHECK(hipStreamSynchronize(L.stream));
-    for (int s = 0; s < S; s++) {
+    // solutions in CPS order: slot s unpacked into the t3 field's memory at
+    // slot s (FP64, per_rhs * 24 doubles each), then copied out
+    double* stage = static_cast<double*>(t3->ref().data);
+    for (int s : done) {
+      double* dst = stage + (size_t)s * o.per_rhs * 24;
+      CPSHIP_CHECK(unpack_spinor(STORE_D, rhs_view(x->ref(), o.per_rhs, s), dst, o.map, L.g.Vcb, L.ls, 1, o.per_rhs,
+                                 L.stream));
       Request* rq = owner[s];
-      if (!rq || act[s]) continue;
+      CPSHIP_CHECK(hipMemcpyAsync(rq->sol + (size_t)rhs_of[s] * o.per_rhs * 24, dst,
+                                  (size_t)o.per_rhs * 24 * sizeof(double), hipMemcpyDeviceToHost, L.stream));
+    }
+    CPSHIP_CHECK(hipStreamSynchronize(L.stream));
+    for (int s : done) {
+      Request* rq = owner[s];
       const int k = rhs_of[s];
-      CPSHIP_CHECK(unpack_spinor(STORE_D, slot_ref(*x, o.per_rhs, s), (double*)L.io.p, o.map, L.g.Vcb, L.ls, 1,
-                                 o.per_rhs, L.stream));
-      CPSHIP_CHECK(hipMemcpyAsync(rq->sol + (size_t)k * o.per_rhs * 24, L.io.p, (size_t)o.per_rhs * 24 * sizeof(double),
-                                  hipMemcpyDeviceToHost, L.stream));
-      CPSHIP_CHECK(hipStreamSynchronize(L.stream));
       cps_hip_cg_result& res = rq->res[k];
       res.iterations = it[s] + 1;
       res.status = st[s];
       res.src_norm_sq = sn[s];
       res.true_res = sn[s] > 0 ? std::sqrt(tr[s] / sn[s]) : 0.0;
-      res.reliable_updates = 0;
+      res.reliable_updates = mixed ? updates[s] : 0;
       if (st[s] == CG_STATUS_D_ZERO) rq->rc = CPS_HIP_ERR_D_ZERO;
       rq->done_rhs++;
       owner[s] = nullptr;
@@ -263,7 +415,11 @@ struct Service {
 
   Engine* find_engine(const Request* rq) {
     for (auto& e : engines)
-      if (e->gauge_id == rq->gauge_id && e->max_iter == rq->max_iter && same_op(e->op, rq->op)) return e.get();
+      if (e->gauge_id == rq->gauge_id && e->max_iter == rq->max_iter && e->prec == rq->precision &&
+          (rq->precision == CPS_HIP_PREC_DOUBLE ||
+           (e->every == rq->reliable_every && e->delta == rq->reliable_delta)) &&
+          same_engine_op(e->op, rq->op))
+        return e.get();
     return nullptr;
   }
 
@@ -272,14 +428,14 @@ struct Service {
   int make_engine(Request* rq, Engine** out) {
     *out = nullptr;
     const GaugeEntry& ge = *gauges[rq->gauge_id];
-    const long long need = Engine::footprint(ge, rq->op, max_slots);
+    const long long need = Engine::footprint(ge, rq->op, max_slots, rq->precision);
     while (used_bytes() + need > budget) {
       auto idle = std::find_if(engines.begin(), engines.end(), [](const std::unique_ptr<Engine>& e) { return !e->busy(); });
       if (idle == engines.end()) return 0;   // wait: no room now
       engines.erase(idle);
     }
     std::unique_ptr<Engine> e(new Engine);
-    int rc = e->setup(ge, rq->gauge_id, rq->op, max_slots, rq->max_iter);
+    int rc = e->setup(ge, rq->gauge_id, *rq, max_slots);
     if (rc) return rc;
     *out = e.get();
     engines.push_back(std::move(e));
@@ -338,8 +494,9 @@ struct Service {
       long long stepped = 0;
       for (auto& e : engines)
         if (e->busy()) {
-          if (int rc = e->step(chunk)) fatal = rc;
-          stepped += chunk;
+          int ran = 0;
+          if (int rc = e->step(chunk, &ran)) fatal = rc;
+          stepped += ran;
         }
       for (auto& e : engines) {
         if (!e->busy()) continue;
@@ -428,15 +585,29 @@ int cps_hip_service_add_gauge(cps_hip_service* s, const int node_sites[4], int l
 
 int cps_hip_service_submit(cps_hip_service* s, int gauge_id, const cps_hip_op* op, const double* src, double* sol,
                            int nrhs, double stop_rsd, int max_num_iter, long long* request_id) {
-  if (!s || !op || !src || !sol || nrhs < 1 || !request_id || max_num_iter < 1) return CPS_HIP_ERR_ARG;
+  cps_hip_cg_arg arg;
+  std::memset(&arg, 0, sizeof(arg));
+  arg.max_num_iter = max_num_iter;
+  arg.stop_rsd = stop_rsd;
+  arg.precision = CPS_HIP_PREC_DOUBLE;
+  return cps_hip_service_submit_arg(s, gauge_id, op, src, sol, nrhs, &arg, request_id);
+}
+
+int cps_hip_service_submit_arg(cps_hip_service* s, int gauge_id, const cps_hip_op* op, const double* src, double* sol,
+                               int nrhs, const cps_hip_cg_arg* arg, long long* request_id) {
+  if (!s || !op || !src || !sol || nrhs < 1 || !request_id || !arg || arg->max_num_iter < 1) return CPS_HIP_ERR_ARG;
+  if (arg->precision < CPS_HIP_PREC_DOUBLE || arg->precision > CPS_HIP_PREC_MIXED_SINGLE) return CPS_HIP_ERR_ARG;
   std::unique_ptr<cpship::Request> rq(new cpship::Request);
   rq->gauge_id = gauge_id;
   rq->op = *op;
   rq->src = src;
   rq->sol = sol;
   rq->nrhs = nrhs;
-  rq->stop_rsd = stop_rsd;
-  rq->max_iter = max_num_iter;
+  rq->stop_rsd = arg->stop_rsd;
+  rq->max_iter = arg->max_num_iter;
+  rq->precision = arg->precision;
+  rq->reliable_every = arg->reliable_every > 0 ? arg->reliable_every : 100;
+  rq->reliable_delta = arg->reliable_delta > 0 ? arg->reliable_delta : 0.1;
   rq->res.resize(nrhs);
   rq->t_submit = clk::now();
   {

@@ -128,16 +128,19 @@ struct Lattice {
 // are captured (no memset nodes).  All kernel arguments are device pointers
 // or values fixed while the operands stay the same, so one capture serves
 // every iteration until reset() (the CG's active-rhs window shrinking).
+// The process-wide controls (cps_hip_tuning): the field's value, or the
+// environment variable CPS_HIP_<NAME> when that is set (read at every call,
+// so a variable set between solves takes effect).
+cps_hip_tuning tuning();
+int knob(int cps_hip_tuning::*field, const char* env);
+
 struct IterGraph {
   bool enabled;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  // CPS_HIP_GRAPH=0 launches every iteration eagerly (e.g. under a
-  // profiler that traces kernels but not graph replays)
-  explicit IterGraph(bool on) : enabled(on) {
-    const char* e = getenv("CPS_HIP_GRAPH");
-    if (e && *e == '0') enabled = false;
-  }
+  // use_graph = 0 launches every iteration eagerly (e.g. under a profiler
+  // that traces kernels but not graph replays)
+  explicit IterGraph(bool on) : enabled(on && knob(&cps_hip_tuning::use_graph, "CPS_HIP_GRAPH") != 0) {}
   IterGraph(const IterGraph&) = delete;
   IterGraph& operator=(const IterGraph&) = delete;
   ~IterGraph() {
@@ -190,6 +193,17 @@ struct CGState {
   int alloc(Lattice& L, int nrhs, int hist_len);
 };
 
+// rhs lo onwards of a batched field / of the per-rhs CG scalars (the same
+// planes with the data, norm and scalar pointers advanced)
+FieldRef rhs_view(const FieldRef& f, int64_t per_rhs, int lo);
+CGScalars scalars_view(const CGScalars& s, int lo);
+
+// After a reliable update of the mixed CG (rr_true = FP64 |r|^2 per rhs, rp
+// = <r, p>): rhs meeting the stopping rule finish, the others continue from
+// the FP64 residual with p re-orthogonalised by coef (p += coef r).
+hipError_t mixed_after_update(const CGScalars& cg, const double* rr_true, const double* rp, double* coef, int nrhs,
+                              hipStream_t st);
+
 int op_layout(const cps_hip_op* op, int ls, OpLayout* o, const Geom& g);
 double op_kappa(const cps_hip_op* op);
 int hop(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
@@ -206,9 +220,21 @@ struct HopClover {
   const FieldRef* R = nullptr;
   const double* ralpha = nullptr;
   const int* ractive = nullptr;
+  // mode CLOV_NONE: per-rhs masses (HopParams::kscale with rhs_per_set 1,
+  // HopParams::kwall), device arrays indexed by rhs0 + local rhs
+  const double* kwall = nullptr;
 };
 int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, int target, int dag,
            double scale, const FieldRef& X, double xcoef, double* partial, int nrhs, const HopClover* hc);
+// M and M^dag M of the Wilson or DWF kind with a mass per rhs (device
+// arrays k2 = kappa^2, Wilson, or wall = -2 mf / a5, DWF, indexed by rhs0 +
+// local rhs)
+int apply_m_slots(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                  const FieldRef& t, int dag, double* partial, int nrhs, const double* k2, const double* wall,
+                  int rhs0);
+int apply_mdagm_slots(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                      const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs, const double* k2,
+                      const double* wall, int rhs0);
 int hop_blocks_per_rhs(Lattice& L, const cps_hip_op* op);
 int tuned_launch(Lattice& L, HopParams& P);
 void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, bool recon_ok, int* sp_store, int* hprec);

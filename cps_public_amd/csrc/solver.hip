@@ -366,6 +366,7 @@ int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, con
     P.kscale = hc->kscale;
     P.rhs_per_set = hc->rhs_per_set;
     P.rhs0 = hc->rhs0;
+    P.kwall = hc->kwall;
     if (hc->R) {
       P.rupd = 1;
       P.R = *hc->R;
@@ -386,14 +387,72 @@ int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, con
 // per-chunk partials (4-D operators) do not depend on the block or mapping.
 // CPS_HIP_TUNE=0 switches tuning off (default block, no variants).
 
-static bool tuning_enabled() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("CPS_HIP_TUNE");
-    v = (e && *e == '0') ? 0 : 1;
-  }
-  return v == 1;
+static std::mutex tuning_mu;
+static cps_hip_tuning tuning_fields = {1, 1, 0, 0, 1, 1, 1, 1, 1, 0, 1, 1, 1, 0, 0, 0.0};
+
+static int env_int(const char* name, int v) {
+  const char* e = getenv(name);
+  return (e && *e) ? atoi(e) : v;
 }
+
+cps_hip_tuning tuning() {
+  cps_hip_tuning t;
+  {
+    std::lock_guard<std::mutex> lk(tuning_mu);
+    t = tuning_fields;
+  }
+  t.use_graph = env_int("CPS_HIP_GRAPH", t.use_graph);
+  t.tune = env_int("CPS_HIP_TUNE", t.tune);
+  t.tune_report = env_int("CPS_HIP_TUNE_REPORT", t.tune_report);
+  if (const char* m = getenv("CPS_HIP_HOP_MODE"))
+    t.hop_mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3
+               : !strcmp(m, "rhsgroup") ? 4 : 0;
+  t.rhs_window = env_int("CPS_HIP_RHS_WINDOW", t.rhs_window);
+  t.lagged_x = env_int("CPS_HIP_LAGGED_X", t.lagged_x);
+  t.paired_x = env_int("CPS_HIP_PAIRED_X", t.paired_x);
+  t.s5_xpay = env_int("CPS_HIP_S5_XPAY", t.s5_xpay);
+  t.r_in_hop = env_int("CPS_HIP_R_IN_HOP", t.r_in_hop);
+  t.clover_r_in_hop = env_int("CPS_HIP_CLOVER_R_IN_HOP", t.clover_r_in_hop);
+  t.m5_fused = env_int("CPS_HIP_M5_FUSED", t.m5_fused);
+  t.s5_fused = env_int("CPS_HIP_S5_FUSED", t.s5_fused);
+  t.s5x_row = env_int("CPS_HIP_S5X_ROW", t.s5x_row);
+  if (const char* e = getenv("CPS_HIP_SLOPPY_VECTORS"))
+    t.sloppy_vectors = !strcmp(e, "single") ? 1 : !strcmp(e, "xhalf") ? 2 : !strcmp(e, "half") ? 3 : 0;
+  t.verbose = env_int("CPS_HIP_VERBOSE", t.verbose);
+  if (const char* e = getenv("CPS_HIP_PROGRESS")) t.progress_seconds = atof(e);
+  return t;
+}
+
+// one integer control: the field, or its environment override
+int knob(int cps_hip_tuning::*field, const char* env) {
+  int v;
+  {
+    std::lock_guard<std::mutex> lk(tuning_mu);
+    v = tuning_fields.*field;
+  }
+  return env_int(env, v);
+}
+
+}  // namespace cpship
+
+extern "C" int cps_hip_get_tuning(cps_hip_tuning* t) {
+  if (!t) return CPS_HIP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(cpship::tuning_mu);
+  *t = cpship::tuning_fields;
+  return 0;
+}
+
+extern "C" int cps_hip_set_tuning(const cps_hip_tuning* t) {
+  if (!t || t->hop_mode < 0 || t->hop_mode > 4 || t->sloppy_vectors < 0 || t->sloppy_vectors > 3)
+    return CPS_HIP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(cpship::tuning_mu);
+  cpship::tuning_fields = *t;
+  return 0;
+}
+
+namespace cpship {
+
+static bool tuning_enabled() { return knob(&cps_hip_tuning::tune, "CPS_HIP_TUNE") != 0; }
 
 // whether a launch may use HOP_SPATIAL at block size b
 static bool spatial_ok(Lattice& L, const HopParams& P, int b) {
@@ -430,13 +489,7 @@ static bool lds_links_ok(const HopParams& P, int b) {
 
 // CPS_HIP_TUNE_REPORT=1 prints each tuning decision to stderr, =2 also the
 // time of every candidate
-static int tune_report() {
-  static const int v = [] {
-    const char* e = getenv("CPS_HIP_TUNE_REPORT");
-    return e ? atoi(e) : 0;
-  }();
-  return v;
-}
+static int tune_report() { return knob(&cps_hip_tuning::tune_report, "CPS_HIP_TUNE_REPORT"); }
 
 int tuned_launch(Lattice& L, HopParams& P) {
   P.nbr = (const uint4*)L.nbr.p;
@@ -447,7 +500,19 @@ int tuned_launch(Lattice& L, HopParams& P) {
   const int def_block = pick_block(per_rhs);
   if (P.kind == KIND_WILSON && L.perm_state >= 0 && L.g.Vcb % 8 == 0 && L.ensure_perm() == 0)
     P.perm = (const uint32_t*)L.perm.p;
-  int mode = 0;
+  if (P.clov == CLOV_KS) {
+    // a mass per rhs: one kernel in element order (no variants to time)
+    P.block = def_block;
+    P.flags = 0;
+    CPSHIP_CHECK(launch_hop(P, L.stream));
+    return 0;
+  }
+  // hop_mode: the field, or CPS_HIP_HOP_MODE=element|sitewave|lds|rhsgroup
+  int mode;
+  {
+    std::lock_guard<std::mutex> lk(tuning_mu);
+    mode = tuning_fields.hop_mode;
+  }
   if (const char* m = getenv("CPS_HIP_HOP_MODE"))
     mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3 : !strcmp(m, "rhsgroup") ? 4 : 0;
   HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr,
@@ -584,6 +649,39 @@ int apply_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, co
   return hop(L, op, hprec, out, t, 1, dag, -k * k, in, 1.0, partial, nrhs);
 }
 
+// apply_m with a mass per rhs (Wilson and DWF kinds): rhs r of the launch
+// uses kappa^2 = k2[rhs0 + r] (Wilson: the second hop's scale) or the wall
+// coefficient wall[rhs0 + r] = -2 mf / a5 (DWF: both hops; kappa_5 does not
+// depend on the mass), device arrays.
+int apply_m_slots(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                  const FieldRef& t, int dag, double* partial, int nrhs, const double* k2, const double* wall,
+                  int rhs0) {
+  HopClover h{};
+  h.mode = CLOV_KS;
+  h.rhs_per_set = 1;
+  h.rhs0 = rhs0;
+  if (op->kind == CPS_HIP_OP_WILSON) {
+    int rc = hop(L, op, hprec, t, in, 0, dag, 1.0, null_field(), 0.0, nullptr, nrhs);
+    if (rc) return rc;
+    h.kscale = k2;
+    return hop_ex(L, op, hprec, out, t, 1, dag, -1.0, in, 1.0, partial, nrhs, &h);
+  }
+  if (op->kind != CPS_HIP_OP_DWF) return CPS_HIP_ERR_ARG;
+  const double k = op_kappa(op);
+  h.kwall = wall;
+  int rc = hop_ex(L, op, hprec, t, in, 0, dag, 1.0, null_field(), 0.0, nullptr, nrhs, &h);
+  if (rc) return rc;
+  return hop_ex(L, op, hprec, out, t, 1, dag, -k * k, in, 1.0, partial, nrhs, &h);
+}
+
+int apply_mdagm_slots(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
+                      const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs, const double* k2,
+                      const double* wall, int rhs0) {
+  int rc = apply_m_slots(L, op, hprec, t2, in, t1, 0, partial, nrhs, k2, wall, rhs0);
+  if (rc) return rc;
+  return apply_m_slots(L, op, hprec, out, t2, t1, 1, nullptr, nrhs, k2, wall, rhs0);
+}
+
 // MatPcDagMatPc: out = M^dag M in; partial (optional) = |M in|^2 per block
 int apply_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in,
                 const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs, bool t1_ready) {
@@ -674,14 +772,7 @@ int CGState::alloc(Lattice& L, int nrhs_, int hist_len_) {
 
 // CPS_HIP_VERBOSE=1 prints the residual history to stderr (the VRB.Flow
 // lines of inv_cg.C:352-353)
-static bool verbose() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = getenv("CPS_HIP_VERBOSE");
-    v = (e && *e && *e != '0') ? 1 : 0;
-  }
-  return v == 1;
-}
+static bool verbose() { return knob(&cps_hip_tuning::verbose, "CPS_HIP_VERBOSE") != 0; }
 
 // CPS_HIP_PROGRESS=<seconds> prints one line to stderr at most every that
 // many seconds while a solve runs (iteration, rhs still active, elapsed).
@@ -690,8 +781,7 @@ struct Progress {
   std::chrono::steady_clock::time_point t0, last;
   const char* what;
   explicit Progress(const char* w) : what(w) {
-    const char* e = getenv("CPS_HIP_PROGRESS");
-    every = e ? atof(e) : 0.0;
+    every = tuning().progress_seconds;
     t0 = last = std::chrono::steady_clock::now();
   }
   // rr: device |r|^2 per rhs (the largest is printed)
@@ -735,19 +825,16 @@ static int auto_check_every(const OpLayout& o, int nrhs) {
 // do not change; converged rhs in the middle of the range keep their masked
 // updates.  CPS_HIP_RHS_WINDOW=0 keeps the whole batch.
 
-static bool rhs_window_enabled() {
-  const char* e = getenv("CPS_HIP_RHS_WINDOW");   // read per solve
-  return !(e && *e == '0');
-}
+static bool rhs_window_enabled() { return knob(&cps_hip_tuning::rhs_window, "CPS_HIP_RHS_WINDOW") != 0; }
 
-static FieldRef rhs_view(const FieldRef& f, int64_t per_rhs, int lo) {
+FieldRef rhs_view(const FieldRef& f, int64_t per_rhs, int lo) {
   FieldRef v = f;
   if (v.data) v.data = static_cast<char*>(v.data) + (int64_t)lo * per_rhs * 16;   // 16-byte chunks
   if (v.norm) v.norm += (int64_t)lo * per_rhs;
   return v;
 }
 
-static CGScalars scalars_view(const CGScalars& s, int lo) {
+CGScalars scalars_view(const CGScalars& s, int lo) {
   CGScalars v = s;
   v.rr += lo;
   v.rr_prev += lo;
@@ -938,11 +1025,12 @@ static void sloppy_vector_storage(int sps, int* xst, int* rst) {
   *xst = *rst = STORE_F;
   if (sps != STORE_H) return;
   *rst = STORE_H;
-  const char* e = getenv("CPS_HIP_SLOPPY_VECTORS");
-  if (!e) return;
-  if (std::strcmp(e, "single") == 0) *rst = STORE_F;
-  else if (std::strcmp(e, "half") == 0) *xst = *rst = STORE_H;
-  else if (std::strcmp(e, "xhalf") == 0) { *xst = STORE_H; *rst = STORE_F; }
+  switch (tuning().sloppy_vectors) {
+    case 1: *rst = STORE_F; break;
+    case 2: *xst = STORE_H; *rst = STORE_F; break;
+    case 3: *xst = *rst = STORE_H; break;
+    default: break;
+  }
 }
 
 static void mixed_formats(const Lattice& L, int prec, const cps_hip_op* op, int nrhs, int* sp_store, int* hprec) {
@@ -984,6 +1072,12 @@ __global__ void k_mixed_after_update(CGScalars cg, const double* rr_true, const 
   cg.rr[k] = v;
   cg.active[k] = 1;
   cg.status[k] = CG_STATUS_RUNNING;
+}
+
+hipError_t mixed_after_update(const CGScalars& cg, const double* rr_true, const double* rp, double* coef, int nrhs,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_mixed_after_update, dim3((nrhs + 63) / 64), dim3(64), 0, st, cg, rr_true, rp, coef, nrhs);
+  return hipGetLastError();
 }
 
 int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int nrhs, const cps_hip_cg_arg* arg,
@@ -1054,35 +1148,25 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   // kernel that reads p next (the direction update), which saves one read of
   // p per iteration; a reliable update applies the pending step first
   // (CPS_HIP_LAGGED_X=0: in the same iteration, with r)
-  const bool lagged = [] {
-    const char* e = getenv("CPS_HIP_LAGGED_X");
-    return !(e && *e == '0');
-  }();
+  const cps_hip_tuning tn = tuning();
+  const bool lagged = tn.lagged_x != 0;
   // Moebius, FP16 directions, Ls 16: the direction update also writes the
   // first step of M, b p + c P p, into the operator's first scratch field
   // (CPS_HIP_S5_XPAY=0: the operator computes it)
-  const bool s5_in_xpay = [&] {
-    const char* e = getenv("CPS_HIP_S5_XPAY");
-    return lagged && !(e && *e == '0') && op->kind == CPS_HIP_OP_MOBIUS && sps == STORE_H && xvec == STORE_F &&
-           L.ls == 16;
-  }();
+  const bool s5_in_xpay = lagged && tn.s5_xpay != 0 && op->kind == CPS_HIP_OP_MOBIUS && sps == STORE_H &&
+                          xvec == STORE_F && L.ls == 16;
   // with it, the residual step r -= alpha A p runs in the epilogue of the
   // last hop of M^dag (alpha is known after the first half of M^dag M), so A p
   // is never stored (CPS_HIP_R_IN_HOP=0: a separate pass)
-  const bool r_in_hop = [&] {
-    const char* e = getenv("CPS_HIP_R_IN_HOP");
-    return s5_in_xpay && svec == STORE_H && !(e && *e == '0');
-  }();
+  const bool r_in_hop = s5_in_xpay && svec == STORE_H && tn.r_in_hop != 0;
   // clover, FP16 directions and residual, on request
   // (CPS_HIP_CLOVER_R_IN_HOP=1): likewise the residual step in the epilogue
   // of the last hop of MatPcDag, after the alpha of the first half.  Off by
   // default: that epilogue (clover multiply, X and r in registers) spills
   // at the hop's register budget and the C4 solve takes 3.85 instead of
   // 2.56 s.
-  const bool clover_rih = [&] {
-    const char* e = getenv("CPS_HIP_CLOVER_R_IN_HOP");
-    return lagged && op->kind == CPS_HIP_OP_CLOVER && sps == STORE_H && svec == STORE_H && e && *e == '1';
-  }();
+  const bool clover_rih = lagged && op->kind == CPS_HIP_OP_CLOVER && sps == STORE_H && svec == STORE_H &&
+                          tn.clover_r_in_hop != 0;
   Field& t3s = L.field_min(S_T3S, sps, n);
   if (L.last_rc) return L.last_rc;
   // FP16 directions and an FP32 solution: iterations run in pairs with the
@@ -1090,10 +1174,7 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   // iteration with both pending steps, in order (bit-identical to one step
   // per iteration, one pass over x less per pair; CPS_HIP_PAIRED_X=0 turns
   // it off)
-  const bool paired = [&] {
-    const char* e = getenv("CPS_HIP_PAIRED_X");
-    return lagged && sps == STORE_H && xvec == STORE_F && !(e && *e == '0');
-  }();
+  const bool paired = lagged && sps == STORE_H && xvec == STORE_F && tn.paired_x != 0;
   Field* ps2 = nullptr;
   if (paired) {
     ps2 = &L.field(S_PS2, sps, n);

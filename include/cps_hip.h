@@ -98,6 +98,34 @@ typedef struct {
   double seconds;        /* wall time of the batched solve (same for every rhs) */
 } cps_hip_cg_result;
 
+/* ---- solver and launch controls ------------------------------------------
+ * Process-wide switches of the solver's fused steps and of the hop launcher.
+ * The defaults are the measured fastest path; each switch exists to compare
+ * a fusion against its unfused form (the results of the two are
+ * bit-identical except where noted) or to restrict the autotuner.  An
+ * environment variable CPS_HIP_<NAME> (e.g. CPS_HIP_LAGGED_X=0), when set,
+ * overrides the field of the same name.  No counterpart in CPS. */
+typedef struct {
+  int use_graph;        /* 1: CG iterations replayed as a captured hipGraph (ANDed with cps_hip_cg_arg.use_graph) */
+  int tune;             /* 1: the hop autotuner times its candidates; 0: default launch */
+  int tune_report;      /* 1: print the tuner's choices, 2: also every candidate's time */
+  int hop_mode;         /* restrict the hop's thread mapping: 0 any, 1 element, 2 sitewave, 3 lds, 4 rhsgroup */
+  int rhs_window;       /* 1: kernels run on the contiguous range of rhs still iterating */
+  int lagged_x;         /* mixed CG: solution step applied by the next direction update */
+  int paired_x;         /* mixed CG: two solution steps per pass over x (FP16 directions) */
+  int s5_xpay;          /* mixed Moebius Ls 16: first fifth-dimension step of M in the direction update */
+  int r_in_hop;         /* mixed Moebius Ls 16: residual step in the last hop of M^dag (rounds differently) */
+  int clover_r_in_hop;  /* mixed clover: residual step in the last hop of MatPcDag (default 0: it spills) */
+  int m5_fused;         /* half precision Ls 16: M5^-1 in the first hop's epilogue */
+  int s5_fused;         /* Moebius M^dag: last fifth-dimension step in the hop epilogue */
+  int s5x_row;          /* FP16 Ls 16: slots s+-1 of the s-term epilogue by DPP row rotations */
+  int sloppy_vectors;   /* half precision: 0 x FP32 + r FP16, 1 both FP32, 2 x FP16 + r FP32, 3 both FP16 */
+  int verbose;          /* 1: per-chunk CG diagnostics on stderr */
+  double progress_seconds; /* > 0: a progress line on stderr at most this often during long solves */
+} cps_hip_tuning;
+int cps_hip_get_tuning(cps_hip_tuning *t);
+int cps_hip_set_tuning(const cps_hip_tuning *t);
+
 /* ---- lattice handle ------------------------------------------------------
  * Replaces wilson_init/dwf_init (wilson_init.C:52-112, dwf_init.C:41-99): it
  * holds the local geometry, the device scratch fields and the resident gauge.
@@ -277,11 +305,12 @@ int cps_hip_bench_copy(cps_hip_lattice *lat, long long n_elem, int reps, double 
 
 /* ---- resident solve service -----------------------------------------------
  * Keeps gauge fields on the device and serves propagator solves by
- * continuous batching: every (lattice, operator) pair in use gets an engine,
- * a batched FP64 CG (inv_cg.C:293-364 per slot) over max_slots right-hand
- * side slots on its own stream; a slot whose rhs converged is harvested and
- * refilled with the next waiting rhs while the others iterate.  Engines of
- * different lattices/masses run concurrently; an engine is created only while
+ * continuous batching: every (lattice, operator, precision) in use gets an
+ * engine, a batched CG (FP64: inv_cg.C:293-364 per slot; or mixed precision
+ * with FP64 reliable updates) over max_slots right-hand side slots on its own
+ * stream; a slot whose rhs converged is harvested and refilled with the next
+ * waiting rhs while the others iterate.  Wilson and DWF engines take a mass
+ * per slot.  Engines run concurrently; an engine is created only while
  * the estimated device footprint fits hbm_budget_bytes (idle engines are
  * released first).  submit() returns at once; src and sol (host, CPS layout,
  * nrhs vectors) must stay valid until wait() returns.  No counterpart in CPS. */
@@ -303,6 +332,15 @@ int cps_hip_service_add_gauge(cps_hip_service *s, const int node_sites[4], int l
                               const double *u, int *gauge_id);
 int cps_hip_service_submit(cps_hip_service *s, int gauge_id, const cps_hip_op *op, const double *src, double *sol,
                            int nrhs, double stop_rsd, int max_num_iter, long long *request_id);
+/* The same with the solver controls of cps_hip_cg_arg: precision (FP64 or a
+ * mixed mode: the sloppy CG of that mode with FP64 reliable updates at least
+ * every reliable_every iterations and when a slot's sloppy residual falls
+ * below reliable_delta times its last FP64 residual; a slot finishes only on
+ * its FP64 residual), stop_rsd, max_num_iter.  Requests of one lattice and
+ * operator kind share an engine when precision and reliable-update controls
+ * agree; Wilson and DWF requests of different masses share it too. */
+int cps_hip_service_submit_arg(cps_hip_service *s, int gauge_id, const cps_hip_op *op, const double *src,
+                               double *sol, int nrhs, const cps_hip_cg_arg *arg, long long *request_id);
 /* Blocks until the request is done; res has nrhs entries; latency_s is the
  * time from submit to the last rhs harvested. */
 int cps_hip_service_wait(cps_hip_service *s, long long request_id, cps_hip_cg_result *res, double *latency_s);

@@ -32,6 +32,10 @@ extern "C" {
 #endif
 
 /* ---- layout ------------------------------------------------------------ */
+/* thread count of the OpenMP site loops; results do not depend on it */
+void ora_set_threads(int n);
+int ora_get_threads(void);
+
 void ora_gauge_canon_to_wilson(double *dst, const double *src, const int L[4]);
 void ora_gauge_wilson_to_canon(double *dst, const double *src, const int L[4]);
 void ora_ferm_canon_to_wilson(double *dst, const double *src, const int L[4]);

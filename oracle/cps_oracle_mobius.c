@@ -500,27 +500,33 @@ static void inv6_ldl(double *A) {
 void ora_clover_mat_chkb(double *A, const double *u, int cb, int inverse,
                          double kappa, double csw, const int L[4]) {
   double omega = kappa * csw;
-  int site[4], vol = vol4(L);
-  double *Ap = A;
-  for (site[3] = 0; site[3] < L[3]; ++site[3])
+  int t, vol = vol4(L);
+  /* the sites of parity cb in the order x/2 + L0/2 (y + L1 (z + L2 t)) */
+#pragma omp parallel for schedule(static)
+  for (t = 0; t < L[3]; ++t) {
+    int site[4];
+    site[3] = t;
     for (site[2] = 0; site[2] < L[2]; ++site[2])
       for (site[1] = 0; site[1] < L[1]; ++site[1])
         for (site[0] = (site[3] + site[2] + site[1] + cb) % 2; site[0] < L[0]; site[0] += 2) {
-          site_clover(Ap, u, site, omega, L);
-          Ap += 72;
+          size_t idx = (size_t)(site[0] / 2) + (size_t)(L[0] / 2) * (site[1] + (size_t)L[1] * (site[2] + (size_t)L[2] * t));
+          site_clover(A + 72 * idx, u, site, omega, L);
         }
+  }
   if (inverse) {
     int i;
-    for (i = 0; i < vol; i++) inv6_ldl(A + 36 * i);
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < vol; i++) inv6_ldl(A + 36 * (size_t)i);
   }
 }
 
 /* clover_mat_mlt_C, clover.C:127-172: Y = A X per 6-component half spinor. */
 void ora_clover_mat_mlt(double *out, const double *mat, const double *in, int n) {
-  double *Y = out;
-  const double *A = mat, *X = in;
   int i;
-  for (i = 0; i < n * 2; ++i, A += 36, Y += 12, X += 12) {
+#pragma omp parallel for schedule(static)
+  for (i = 0; i < n * 2; ++i) {
+    double *Y = out + 12 * (size_t)i;
+    const double *A = mat + 36 * (size_t)i, *X = in + 12 * (size_t)i;
     Y[0] = A[0]*X[0] + A[1]*X[2] + A[2]*X[3] + A[4]*X[4] + A[5]*X[5] + A[9]*X[6] + A[10]*X[7] + A[16]*X[8] + A[17]*X[9] + A[25]*X[10] + A[26]*X[11];
     Y[1] = A[0]*X[1] + A[1]*X[3] - A[2]*X[2] + A[4]*X[5] - A[5]*X[4] + A[9]*X[7] - A[10]*X[6] + A[16]*X[9] - A[17]*X[8] + A[25]*X[11] - A[26]*X[10];
     Y[2] = A[1]*X[0] - A[2]*X[1] + A[3]*X[2] + A[6]*X[4] + A[7]*X[5] + A[11]*X[6] + A[12]*X[7] + A[18]*X[8] + A[19]*X[9] + A[27]*X[10] + A[28]*X[11];

@@ -9,6 +9,7 @@
 #include "cps_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -16,6 +17,10 @@
 #define GAUGE 72
 
 static int vol4(const int L[4]) { return L[0] * L[1] * L[2] * L[3]; }
+
+/* Thread count of the OpenMP site loops (results do not depend on it). */
+void ora_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int ora_get_threads(void) { return omp_get_max_threads(); }
 
 /* ======================================================================
  * Layout.  Follows convert_func.C:318-415 (gauge CANONICAL->WILSON,
@@ -316,10 +321,14 @@ void ora_wilson_dslash(double *chi_p, const double *u_p, const double *psi_p,
   int vol = vol4(L) / 2; /* checkerboard volume, Wilson.vol[0] */
   int sdag = (dag == 0) ? 1 : -1;
   int cbn = (cb == 0) ? 1 : 0;
-  double tmp[SPINOR], temps[4][2][SPINOR];
-  int x, y, z, t, mu, i;
+  int x, y;
+  /* Sites are independent (each writes its own chi): OpenMP threads only
+   * split the loop, the arithmetic of every site is unchanged. */
+#pragma omp parallel for collapse(2) schedule(static)
   for (x = 0; x < lx; x++)
-    for (y = 0; y < ly; y++)
+    for (y = 0; y < ly; y++) {
+      double tmp[SPINOR], temps[4][2][SPINOR];
+      int z, t, mu, i;
       for (z = 0; z < lz; z++)
         for (t = 0; t < lt; t++) {
           if ((x + y + z + t) % 2 != cbn) continue;
@@ -350,6 +359,7 @@ void ora_wilson_dslash(double *chi_p, const double *u_p, const double *psi_p,
             chi[i] = (temps[0][0][i] + temps[1][0][i] + temps[2][0][i] + temps[3][0][i] +
                       temps[0][1][i] + temps[1][1][i] + temps[2][1][i] + temps[3][1][i]);
         }
+    }
 }
 
 /* wilson_m.C:49-93 */

@@ -52,6 +52,8 @@ class OraMobius(ctypes.Structure):
 def _declare(l):
     vp = ctypes.c_void_p
     sig = {
+        "ora_set_threads": [ctypes.c_int],
+        "ora_get_threads": [],
         "ora_gauge_canon_to_wilson": [vp, vp, vp],
         "ora_gauge_wilson_to_canon": [vp, vp, vp],
         "ora_ferm_canon_to_wilson": [vp, vp, vp],
@@ -92,7 +94,7 @@ def _declare(l):
     for name, args in sig.items():
         f = getattr(l, name)
         f.argtypes = args
-        f.restype = ctypes.c_int if name.endswith(("invcg", "matinv")) else None
+        f.restype = ctypes.c_int if name.endswith(("invcg", "matinv", "get_threads")) else None
     l.ora_dot.argtypes = [vp, vp, ctypes.c_size_t]
     l.ora_dot.restype = ctypes.c_double
     l.ora_ftimesv1plusv2.argtypes = [vp, ctypes.c_double, vp, vp, ctypes.c_size_t]
@@ -177,6 +179,16 @@ def bond_cond_twist(u_canon, L, bc, twist, undo=False):
     tw = np.ascontiguousarray(np.asarray(twist, dtype=np.float64))
     lib().ora_bond_cond_twist(_p(u), _L(L), _L(bc), _p(tw), int(bool(undo)))
     return u
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's site loops (the results do not depend
+    on the count)."""
+    lib().ora_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().ora_get_threads())
 
 
 # ---- vector BLAS ---------------------------------------------------------

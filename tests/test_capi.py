@@ -39,3 +39,22 @@ def test_null_handle_is_an_argument_error():
     assert l.cps_hip_lattice_destroy(None) == -1
     assert l.cps_hip_gauge_import(None, None) == -1
     assert l.cps_hip_wilson_dslash(None, None, None, None, 0, 0) == -1
+
+
+def test_tuning_controls_round_trip():
+    """cps_hip_set_tuning / cps_hip_get_tuning (no device needed): the fields
+    read back as set, out-of-range values are refused, the defaults come
+    back."""
+    base = C.get_tuning()
+    assert (base.use_graph, base.tune, base.lagged_x, base.paired_x, base.clover_r_in_hop) == (1, 1, 1, 1, 0)
+    t = C.get_tuning()
+    t.paired_x, t.hop_mode, t.progress_seconds = 0, 2, 5.0
+    C.set_tuning(t)
+    got = C.get_tuning()
+    assert (got.paired_x, got.hop_mode, got.progress_seconds) == (0, 2, 5.0)
+    bad = C.get_tuning()
+    bad.hop_mode = 9
+    with pytest.raises(C.CpsHipError):
+        C.set_tuning(bad)
+    C.set_tuning(base)
+    assert C.get_tuning().paired_x == 1

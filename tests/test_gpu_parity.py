@@ -715,6 +715,57 @@ def test_service_results_same_alone_and_in_mix(oracle):
     svc.close()
 
 
+@pytest.mark.parametrize("prec", [C.PREC_MIXED_HALF, C.PREC_MIXED_SINGLE])
+def test_service_mixed_precision_shared_masses(oracle, prec):
+    """Mixed-precision requests (sloppy CG with FP64 reliable updates per
+    slot): Wilson and DWF requests of different masses share one engine per
+    lattice; every rhs meets the FP64 true residual, matches the FP64 oracle
+    CG, and agrees alone and in the mix within the solver tolerance (reliable
+    updates of an engine fall on the same iterations for all its slots, so
+    the mix changes when a slot's updates happen, not what it converges
+    to)."""
+    LW, LD, ls = [8, 8, 8, 8], [4, 4, 4, 8], 8
+    uw = wilson_gauge(LW, 0.2, 160)
+    ud = wilson_gauge(LD, 0.2, 161)
+    svc = C.Service(8 << 30, max_slots=4)
+    gw = svc.add_gauge(LW, 1, uw)
+    gd = svc.add_gauge(LD, ls, ud)
+    vw, vd = 24 * int(np.prod(LW)) // 2, 24 * int(np.prod(LD)) // 2 * ls
+    reqs = []
+    for k, (g, mass, nrhs) in enumerate([(gw, 0.1, 2), (gd, 0.02, 3), (gw, 0.2, 3), (gd, 0.05, 2), (gw, 0.05, 1)]):
+        kind = C.OP_WILSON if g == gw else C.OP_DWF
+        op = C.make_op(kind, mass=mass, m5=1.8)
+        nv = vw if g == gw else vd
+        src = np.concatenate([F.gaussian_fermion(nv // 24, 170 + 10 * k + r) for r in range(nrhs)])
+        reqs.append((g, op, src, nrhs, mass))
+    alone = []
+    for g, op, src, nrhs, _ in reqs:
+        alone.append(svc.wait(svc.submit(g, op, src, nrhs, precision=prec, reliable_every=50)))
+    created_alone = svc.stats()["engines_created"]
+    ids = [svc.submit(g, op, src, nrhs, precision=prec, reliable_every=50) for g, op, src, nrhs, _ in reqs]
+    mix = [svc.wait(rid) for rid in ids]
+    st = svc.stats()
+    print("service mixed stats", prec, st, flush=True)
+    for (g, op, src, nrhs, mass), (sa, ra, la), (sm, rm, lm) in zip(reqs, alone, mix):
+        print("service mixed request", g, mass, [r.iterations for r in rm], [r.reliable_updates for r in rm],
+              max(r.true_res for r in rm), rel(sm, sa), flush=True)
+        assert max(r.true_res for r in ra) <= 1e-10 and max(r.true_res for r in rm) <= 1e-10
+        assert all(r.status == 1 for r in rm)
+        assert rel(sm, sa) < 1e-8
+        n = len(src) // nrhs
+        for r in range(nrhs):
+            if g == gw:
+                ref, _, _ = oracle.wilson_invcg(uw, src[r * n:(r + 1) * n], mass, 1e-12, 5000, LW)
+            else:
+                ref, _, _ = oracle.dwf_invcg(ud, src[r * n:(r + 1) * n], mass, 1e-12, 5000,
+                                             oracle.dwf_struct(LD, ls, 1.8))
+            assert rel(sm[r * n:(r + 1) * n], ref) < 1e-8
+    # one engine per lattice serves all of its masses
+    assert created_alone <= 2 and st["engines_created"] <= 2
+    assert st["last_error"] == 0
+    svc.close()
+
+
 def test_half_storage_subnormal_site():
     """A site whose values are FP32 subnormals (as far from a point source on
     a large lattice) is stored as zero in FP16 storage, not as inf/NaN: the
