@@ -22,7 +22,8 @@ PREC_DOUBLE, PREC_MIXED_HALF, PREC_MIXED_BF16, PREC_MIXED_SINGLE = 0, 1, 2, 3
 ERR_NAMES = {-1: "bad argument", -2: "HIP runtime failure", -3: "no gauge field resident",
              -4: "CG d == 0 (inv_cg.C:326-329)", -5: "device allocation failed", -6: "no HIP device",
              -7: "file missing, truncated or malformed", -8: "NERSC checksum mismatch",
-             -9: "NERSC plaquette mismatch"}
+             -9: "NERSC plaquette mismatch", -10: "request exceeds the no-allocation scratch array"}
+ERR_TOO_LARGE = -10
 
 
 class CpsHipError(RuntimeError):
@@ -105,6 +106,7 @@ def _declare(l):
         "cps_hip_dwf_mdagm": [vp, vp, vp, vp, vp, d, d, d],
         "cps_hip_matpc_dag_matpc": [vp, P(Op), vp, vp, vp, i],
         "cps_hip_matpc": [vp, P(Op), vp, vp, i, i],
+        "cps_hip_matpc_dag_matpc_noalloc": [vp, P(Op), vp, vp, vp, i],
         "cps_hip_matpc_dag_matpc_prec": [vp, P(Op), i, vp, vp, vp, i],
         "cps_hip_clover_build": [vp, vp, i, d],
         "cps_hip_clover_export": [vp, i, i, vp],
@@ -138,6 +140,8 @@ def _declare(l):
         f.argtypes = args
         f.restype = ctypes.c_int
     l.cps_hip_version.restype = ctypes.c_char_p
+    l.cps_hip_noalloc_scratch_bytes.argtypes = []
+    l.cps_hip_noalloc_scratch_bytes.restype = ctypes.c_size_t
     l.cps_hip_lattice_device_bytes.argtypes = [vp]
     l.cps_hip_lattice_device_bytes.restype = ctypes.c_longlong
 
@@ -150,6 +154,11 @@ def _check(rc, what):
 def _ptr(a):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def noalloc_scratch_bytes():
+    """Size of the static device array of cps_hip_matpc_dag_matpc_noalloc."""
+    return int(lib().cps_hip_noalloc_scratch_bytes())
 
 
 def get_tuning():
@@ -283,6 +292,16 @@ class Lattice:
         dp = np.zeros(nrhs)
         _check(lib().cps_hip_matpc_dag_matpc(self.h, ctypes.byref(op), _ptr(out), _ptr(inp), _ptr(dp), nrhs),
                "cps_hip_matpc_dag_matpc")
+        return out, dp
+
+    def mdagm_noalloc(self, op, inp, nrhs=1):
+        """MatPcDagMatPc (Wilson or DWF) through the entry point that
+        allocates no device memory (cps_hip_matpc_dag_matpc_noalloc)."""
+        inp = np.ascontiguousarray(inp, dtype=np.float64)
+        out = np.zeros(self._nreal(op) * nrhs)
+        dp = np.zeros(nrhs)
+        _check(lib().cps_hip_matpc_dag_matpc_noalloc(self.h, ctypes.byref(op), _ptr(out), _ptr(inp), _ptr(dp), nrhs),
+               "cps_hip_matpc_dag_matpc_noalloc")
         return out, dp
 
     def mdagm_prec(self, op, inp, precision, nrhs=1):

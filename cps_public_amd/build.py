@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcps_hip.so")
-SOURCES = ["hop.hip", "hop_clover.hip", "layout.hip", "blas.hip", "solver.hip", "mobius.hip", "clover.hip", "qpropw.hip", "canonical.hip", "capi.cpp", "service.cpp", "nersc.cpp"]
+SOURCES = ["hop.hip", "hop_clover.hip", "layout.hip", "blas.hip", "solver.hip", "mobius.hip", "clover.hip", "qpropw.hip", "canonical.hip", "noalloc.hip", "capi.cpp", "service.cpp", "nersc.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CPS_HIP_ARCH", "gfx950")
 FLAGS = [

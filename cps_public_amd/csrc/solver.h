@@ -237,6 +237,17 @@ int apply_mdagm_slots(Lattice& L, const cps_hip_op* op, int hprec, const FieldRe
                       const double* wall, int rhs0);
 int hop_blocks_per_rhs(Lattice& L, const cps_hip_op* op);
 int tuned_launch(Lattice& L, HopParams& P);
+// While a NoAllocScope is alive on a thread, the hop launcher of that thread
+// allocates nothing and times nothing (it builds no tables and uses a cached
+// or the default launch choice): the no-allocation entry point
+// cps_hip_matpc_dag_matpc_noalloc runs under it.
+extern thread_local int tl_no_alloc;
+struct NoAllocScope {
+  NoAllocScope() { tl_no_alloc++; }
+  ~NoAllocScope() { tl_no_alloc--; }
+  NoAllocScope(const NoAllocScope&) = delete;
+  NoAllocScope& operator=(const NoAllocScope&) = delete;
+};
 void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, bool recon_ok, int* sp_store, int* hprec);
 int apply_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, const FieldRef& in, const FieldRef& t,
             int dag, double* partial, int nrhs);

@@ -491,6 +491,13 @@ static bool lds_links_ok(const HopParams& P, int b) {
 // time of every candidate
 static int tune_report() { return knob(&cps_hip_tuning::tune_report, "CPS_HIP_TUNE_REPORT"); }
 
+// the tuner's winners are kept per process (every handle, engine and thread
+// of the same geometry reuses them); a launch shape is timed once
+static std::mutex tune_mu;
+static std::map<HopTuneKey, HopTuneVal> tune_cache;
+
+thread_local int tl_no_alloc = 0;
+
 int tuned_launch(Lattice& L, HopParams& P) {
   P.nbr = (const uint4*)L.nbr.p;
   P.nbr_neg = (const uint32_t*)L.nbr_neg.p;
@@ -498,6 +505,31 @@ int tuned_launch(Lattice& L, HopParams& P) {
   P.perm_S = L.g.Vcb / 8;
   const long long per_rhs = (long long)P.nsites_out * P.nb;
   const int def_block = pick_block(per_rhs);
+  if (tl_no_alloc) {
+    // no allocation and no timing (cps_hip_matpc_dag_matpc_noalloc): the
+    // choice cached for this launch shape if it needs no table the handle
+    // lacks, else the default launch (results do not depend on the choice)
+    if (P.kind == KIND_WILSON && L.perm_state > 0) P.perm = (const uint32_t*)L.perm.p;
+    HopTuneVal chosen{def_block, 0, 0.0};
+    if (P.clov != CLOV_KS) {
+      const HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr,
+                           P.X.data != nullptr, P.clov + 16 * P.m5post + 32 * P.rupd,
+                           {L.g.L[0], L.g.L[1], L.g.L[2], L.g.L[3]}, 0};
+      std::lock_guard<std::mutex> lk(tune_mu);
+      auto f = tune_cache.find(key);
+      if (f != tune_cache.end()) {
+        const HopTuneVal c = f->second;
+        const bool ok = (!(c.flags & HOP_SPATIAL) || P.perm) &&
+                        (!(c.flags & HOP_SITEWAVE) || sitewave_ok(L, P, c.block, (c.flags & HOP_SPATIAL) != 0)) &&
+                        (!(c.flags & HOP_RHSGROUP) || rhsgroup_ok(P, c.block, c.flags));
+        if (ok) chosen = c;
+      }
+    }
+    P.block = chosen.block;
+    P.flags = chosen.flags;
+    CPSHIP_CHECK(launch_hop(P, L.stream));
+    return 0;
+  }
   if (P.kind == KIND_WILSON && L.perm_state >= 0 && L.g.Vcb % 8 == 0 && L.ensure_perm() == 0)
     P.perm = (const uint32_t*)L.perm.p;
   if (P.clov == CLOV_KS) {
@@ -517,10 +549,6 @@ int tuned_launch(Lattice& L, HopParams& P) {
     mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3 : !strcmp(m, "rhsgroup") ? 4 : 0;
   HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr,
                  P.clov + 16 * P.m5post + 32 * P.rupd, {L.g.L[0], L.g.L[1], L.g.L[2], L.g.L[3]}, mode};
-  // the winners are kept per process (every handle, engine and thread of the
-  // same geometry reuses them); a launch shape is timed once
-  static std::mutex tune_mu;
-  static std::map<HopTuneKey, HopTuneVal> tune_cache;
   HopTuneVal chosen;
   bool have = false;
   {

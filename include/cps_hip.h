@@ -43,6 +43,7 @@ extern "C" {
 #define CPS_HIP_ERR_IO (-7)        /* file missing, truncated or malformed */
 #define CPS_HIP_ERR_CHECKSUM (-8)  /* NERSC checksum mismatch (ReadLatticePar.C:161-176) */
 #define CPS_HIP_ERR_PLAQUETTE (-9) /* NERSC plaquette mismatch (ReadLatticePar.C:283-286) */
+#define CPS_HIP_ERR_TOO_LARGE (-10) /* request exceeds the no-allocation scratch array */
 
 /* operator kinds */
 #define CPS_HIP_OP_WILSON 0
@@ -190,6 +191,20 @@ int cps_hip_dwf_mdagm(cps_hip_lattice *lat, double *out, const double *u, const 
 int cps_hip_matpc_dag_matpc(cps_hip_lattice *lat, const cps_hip_op *op, double *out, const double *in,
                             double *dot_prd, int nrhs);
 int cps_hip_matpc(cps_hip_lattice *lat, const cps_hip_op *op, double *out, const double *in, int dag, int nrhs);
+/* cps_hip_matpc_dag_matpc for the Wilson and DWF operators without any
+ * device allocation.  No counterpart in CPS: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The call's device buffers (staging of in
+ * and out, the operator's fields, reduction partials) live in one static
+ * device array of cps_hip_noalloc_scratch_bytes() (256 MiB) per GPU, held
+ * until the process ends; a request that needs more (five FP64 fields of
+ * nrhs vectors plus partials) returns CPS_HIP_ERR_TOO_LARGE.  The gauge field
+ * must be resident (an import beforehand).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (several threads or handles at
+ * once) race.  Same results as cps_hip_matpc_dag_matpc. */
+int cps_hip_matpc_dag_matpc_noalloc(cps_hip_lattice *lat, const cps_hip_op *op, double *out, const double *in,
+                                    double *dot_prd, int nrhs);
+size_t cps_hip_noalloc_scratch_bytes(void);
 /* MatPcDagMatPc evaluated in the storage and arithmetic of a solver
  * precision mode (CPS_HIP_PREC_*): the FP64 host input is converted to that
  * format on the device, the operator runs there, and the result comes back
