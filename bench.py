@@ -161,7 +161,7 @@ def _oracle_cg(O, cfg, u_w, s0, stop_rsd, max_iter, clov=None):
     return O.wilson_invcg(u_w, s0, cfg["mass"], stop_rsd, max_iter, L)
 
 
-def cpu_baseline(cfg, u_w, src, gpu_iters, full=False):
+def cpu_baseline(cfg, u_w, src, gpu_iters, full=False, legs=None):
     """The CPU restatement (oracle/) of the CPS noarch FP64 CG on the same
     gauge field and rhs 0, on this machine's host cores: with OpenMP over the
     sites of the hopping term at the machine's thread count (the reported
@@ -184,8 +184,10 @@ def cpu_baseline(cfg, u_w, src, gpu_iters, full=False):
                 O.clover_mat_chkb(u_w, 0, 1, kappa, cfg["csw"], L))
     out = {"unit": "s/RHS", "kind": "port", "host": host_info()}
     if full:
+        todo = legs if legs else ([nthr, 1] if nthr > 1 else [1])
+        nthr = todo[0]
         legs = {}
-        for thr in ([nthr, 1] if nthr > 1 else [1]):
+        for thr in todo:
             O.set_threads(thr)
             t0 = time.perf_counter()
             _, it, tr = _oracle_cg(O, cfg, u_w, s0, 1e-10, 100000, clov)
@@ -197,9 +199,10 @@ def cpu_baseline(cfg, u_w, src, gpu_iters, full=False):
                    sample=f"rhs 0 solved to convergence (stop_rsd 1e-10) by the oracle FP64 CG with {nthr} "
                           f"threads: {legs[nthr]['iterations']} iterations, true residual "
                           f"{legs[nthr]['true_res']:.2e}")
-        if 1 in legs and nthr != 1:
-            out["one_thread"] = {"value": legs[1]["value"], "cores": 1,
-                                 "sample": f"the same solve with one thread: {legs[1]['iterations']} iterations"}
+        for thr in todo[1:]:
+            out.setdefault("other_legs", []).append(
+                {"value": legs[thr]["value"], "cores": thr,
+                 "sample": f"the same solve with {thr} thread(s): {legs[thr]['iterations']} iterations"})
         return out
     # bounded sample at full size with nthr threads: n_it CG iterations plus
     # the two extra M^dag M of InvCg (initial residual, true residual)
@@ -242,58 +245,69 @@ SERVE_MASSES = [0.005, 0.01, 0.02, 0.05]
 
 
 def serve_stream(a):
-    """Solve-serving stream: requests with Poisson arrivals, each a Wilson or
-    DWF solve (Ls of the lattice's gauge entry) on one of three resident
-    lattices, a mass from SERVE_MASSES and 1-12 Gaussian sources, solved to
-    1e-10 by the resident service (continuous batching over CG slots).  The
-    latency of a request runs from its arrival (submit) to its last solution
-    being harvested."""
+    """Solve-serving stream: requests with Poisson arrivals at a fixed rate,
+    each a Wilson or DWF propagator solve (Ls of the lattice's gauge entry) on
+    one of three resident lattices, a mass from SERVE_MASSES and 1-12 point
+    sources (spin-colour e_k at a random site on the checkerboard CG works
+    on), solved to FP64 true residual 1e-10 by the resident service
+    (continuous batching over CG slots; mixed precision with FP64 reliable
+    updates unless --serve-fp64).  The latency of a request runs from its
+    arrival (submit) to its last solution being harvested."""
     import threading
     rng = np.random.default_rng(505)
-    svc = C.Service(200 << 30, max_slots=12)
+    svc = C.Service(240 << 30, max_slots=12)
     gids = []
-    for L, ls in (SERVE_LATTICES if a.full else SERVE_LATTICES[:2]):
+    lattices = SERVE_LATTICES if a.full else SERVE_LATTICES[:2]
+    for L, ls in lattices:
         u = F.gauge_field(L, 0.2, 500 + len(gids))
         gids.append(svc.add_gauge(L, ls, LY.gauge_canon_to_wilson(LY.bond_cond(u, L, [0, 0, 0, 1]), L)))
     print("# service: gauges resident", file=sys.stderr, flush=True)
-    # request mix: the 24^3x48 lattice only with --full (its FP64 solves take
-    # minutes), so that the default stream finishes within the bench budget
-    probs = [0.5, 0.4, 0.1] if a.full else [0.6, 0.4, 0.0]
+    # request mix over the lattices: 50 / 40 / 10 % with the 24^3x48 lattice
+    # (--full), 60 / 40 % without it
+    probs = [0.5, 0.4, 0.1] if a.full else [0.6, 0.4]
+    prec = C.PREC_DOUBLE if a.serve_fp64 else C.PREC_MIXED_HALF
     plan = []
     t = 0.0
     for k in range(a.requests):
         t += rng.exponential(1.0 / a.rate)
-        li = int(rng.choice(3, p=probs))
+        li = int(rng.choice(len(probs), p=probs))
         kind = C.OP_WILSON if rng.random() < 0.5 else C.OP_DWF
         mass = float(rng.choice(SERVE_MASSES))
         nrhs = int(rng.integers(1, 13))
-        plan.append((t, li, kind, mass, nrhs))
+        site = int(rng.integers(0, int(np.prod(lattices[li][0])) // 2))
+        plan.append((t, li, kind, mass, nrhs, site))
     lat_s, rhs_total, results = [], 0, []
     lock = threading.Lock()
     waiters = []
 
     def waiter(rid, meta):
         sol, res, lat = svc.wait(rid)
+        del sol
         with lock:
             lat_s.append(lat)
-            results.append((meta, max(r.true_res for r in res), [r.iterations for r in res]))
+            results.append((meta, max(r.true_res for r in res), [r.iterations for r in res], lat))
 
     t0 = time.perf_counter()
-    for (ta, li, kind, mass, nrhs) in plan:
+    for (ta, li, kind, mass, nrhs, site) in plan:
         now = time.perf_counter() - t0
         if ta > now:
             time.sleep(ta - now)
-        L, ls = SERVE_LATTICES[li]
+        L, ls = lattices[li]
         vcb = int(np.prod(L)) // 2
-        per = vcb * (1 if kind == C.OP_WILSON else ls)
-        src = np.concatenate([F.gaussian_fermion(per, 7000 + 13 * len(waiters) + r) for r in range(nrhs)])
+        per = 24 * vcb * (1 if kind == C.OP_WILSON else ls)
+        # point sources e_{k mod 12} at checkerboard site `site` (slice s = 0);
+        # np.zeros pages stay unbacked until written
+        src = np.zeros(per * nrhs)
+        for r in range(nrhs):
+            spin, colour = divmod(r % 12, 3)
+            src[r * per + 24 * site + 2 * (colour + 3 * spin)] = 1.0
         op = C.make_op(kind, mass=mass, m5=1.8)
-        rid = svc.submit(gids[li], op, src, nrhs)
+        rid = svc.submit(gids[li], op, src, nrhs, precision=prec)
         rhs_total += nrhs
         th = threading.Thread(target=waiter, args=(rid, (li, kind, mass, nrhs)))
         th.start()
         waiters.append(th)
-        if len(waiters) % 8 == 0:
+        if len(waiters) % 50 == 0:
             print(f"# service: {len(waiters)} requests submitted, {len(lat_s)} done", file=sys.stderr, flush=True)
     for th in waiters:
         th.join()
@@ -302,16 +316,25 @@ def serve_stream(a):
     svc.close()
     lat = np.array(lat_s)
     worst = max(r[1] for r in results)
+    per_lat = {}
+    for li, (L, ls) in enumerate(lattices):
+        ls_ = [r[3] for r in results if r[0][0] == li]
+        if ls_:
+            per_lat["x".join(str(v) for v in L)] = {"requests": len(ls_), "p50_s": float(np.percentile(ls_, 50)),
+                                                    "p99_s": float(np.percentile(ls_, 99))}
+    names = ", ".join(f"{'x'.join(map(str, L))} (Ls {ls})" for L, ls in lattices)
     line = {
         "metric": "solve-serving stream: p50/p99 latency and solves/s (resident gauge, continuous batching)",
         "value": len(lat) / wall, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
         "requests": len(lat), "rhs": rhs_total, "wall_s": wall, "rate_per_s": a.rate,
         "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-        "rhs_per_s": rhs_total / wall, "true_res_max": worst, "dtype": "f64",
-        "data": "synthetic (seeded SU(3) links exp(i 0.2 H), Gaussian sources)",
-        "config": {"workload": "serve: Wilson/DWF on 8^4 (Ls 8), 16^3x32 (Ls 16)"
-                               + (", 24^3x48 (Ls 16)" if a.full else "") + "; masses "
-                               + ",".join(str(m) for m in SERVE_MASSES) + "; 1-12 rhs; stop_rsd 1e-10"},
+        "rhs_per_s": rhs_total / wall, "true_res_max": worst, "per_lattice": per_lat,
+        "dtype": "f64" if a.serve_fp64 else "f32",
+        "data": "synthetic (seeded SU(3) links exp(i 0.2 H), point sources at random sites)",
+        "config": {"workload": f"serve: Wilson/DWF on {names}; lattice mix " + "/".join(str(p) for p in probs)
+                               + "; masses " + ",".join(str(m) for m in SERVE_MASSES)
+                               + "; 1-12 rhs; stop_rsd 1e-10; "
+                               + ("FP64 CG" if a.serve_fp64 else "mixed FP16/FP32 CG with FP64 reliable updates")},
         "service": st,
     }
     print(json.dumps(line))
@@ -326,13 +349,16 @@ def main():
     ap.add_argument("--requests", type=int, default=48, help="c5: number of requests in the stream")
     ap.add_argument("--rate", type=float, default=1.0, help="c5: Poisson arrival rate (requests/s)")
     ap.add_argument("--full", action="store_true",
-                    help="c5: include the 24^3x48 lattice in the mix (slow in FP64; off by default to keep the "
-                         "default run within minutes)")
+                    help="c5: include the 24^3x48 lattice in the mix (off by default to keep the default run "
+                         "within minutes)")
+    ap.add_argument("--serve-fp64", action="store_true", help="c5: FP64 CG engines instead of mixed precision")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-full", action="store_true",
                     help="CPU leg: solve rhs 0 to convergence with the machine's thread count and with one thread "
                          "(instead of the bounded sample); meant for c1 and c2")
     ap.add_argument("--cpu-only", action="store_true", help="run only the CPU leg (no GPU work)")
+    ap.add_argument("--cpu-threads", default="",
+                    help="with --cpu-full: comma-separated thread counts of the legs (default: the machine's, then 1)")
     ap.add_argument("--no-bf16", action="store_true",
                     help="c3h: skip the extra solve of the same workload with BF16 link storage")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
@@ -361,7 +387,8 @@ def main():
     print("# inputs ready", file=sys.stderr, flush=True)
     if a.cpu_only:
         print(json.dumps({"config": {"workload": cfg["name"]},
-                          "cpu_baseline": cpu_baseline(cfg, u_w, src, 0, full=True)}))
+                          "cpu_baseline": cpu_baseline(cfg, u_w, src, 0, full=True,
+                                                       legs=[int(v) for v in a.cpu_threads.split(",") if v])}))
         return
     lat = C.Lattice(cfg["L"], cfg["ls"], (0, 0, 0, 1))
     lat.import_gauge(u_w)

@@ -1341,8 +1341,10 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
       sync_read(L, al.data(), cg.s.alpha, nrhs * sizeof(double));
       sync_read(L, be.data(), cg.s.beta, nrhs * sizeof(double));
       sync_read(L, dd.data(), cg.s.d, nrhs * sizeof(double));
-      fprintf(stderr, "cps_hip mixed CG iter %d: |r_s|^2 %.6e alpha %.6e beta %.6e d %.6e active %d\n", done, rr[0],
-              al[0], be[0], dd[0], act[0]);
+      fprintf(stderr, "cps_hip mixed CG iter %d window [%d,%d):", done, lo, lo + nw);
+      for (int k = 0; k < nrhs; k++)
+        fprintf(stderr, " rhs %d |r_s|^2 %a alpha %a beta %a d %a active %d;", k, rr[k], al[k], be[k], dd[k], act[k]);
+      fprintf(stderr, "\n");
     }
     if (since >= every || paused || done >= max_itr) {
       // ---- FP64-critical steps: accumulate the solution, recompute r ----
@@ -1376,7 +1378,9 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
       if (verbose()) {
         std::vector<double> rt(nrhs);
         sync_read(L, rt.data(), rr_true, nrhs * sizeof(double));
-        fprintf(stderr, "cps_hip mixed CG reliable update at %d: true |r|^2 %.6e\n", done, rt[0]);
+        fprintf(stderr, "cps_hip mixed CG reliable update at %d: true |r|^2", done);
+        for (int k = 0; k < nrhs; k++) fprintf(stderr, " %a", rt[k]);
+        fprintf(stderr, "\n");
       }
     }
   }
