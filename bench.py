@@ -187,6 +187,7 @@ def cpu_baseline(cfg, u_w, src, gpu_iters, full=False, legs=None):
         todo = legs if legs else ([nthr, 1] if nthr > 1 else [1])
         nthr = todo[0]
         legs = {}
+        O.set_progress(10)   # the solve reports its own progress on stderr
         for thr in todo:
             O.set_threads(thr)
             t0 = time.perf_counter()
@@ -367,6 +368,8 @@ def main():
     ap.add_argument("--hop-reps", type=int, default=50)
     ap.add_argument("--max-iter", type=int, default=20000,
                     help="CG iteration cap (a small cap gives a bounded run for kernel traces)")
+    ap.add_argument("--reliable-every", type=int, default=100,
+                    help="mixed CG: FP64 reliable update at least every this many iterations")
     ap.add_argument("--reliable-delta", type=float, default=None,
                     help="mixed CG: reliable update when |r_sloppy| < delta |r| of the last update (default: the "
                          "workload's value, 0.01 for c3h and c4 and 0.1 otherwise)")
@@ -410,7 +413,7 @@ def main():
         return
     src_dev = C.DeviceBuffer.from_array(src)
     delta = a.reliable_delta if a.reliable_delta is not None else cfg.get("reliable_delta", 0.1)
-    arg_main = C.make_cg_arg(1e-10, a.max_iter, precision=prec_main, reliable_every=100,
+    arg_main = C.make_cg_arg(1e-10, a.max_iter, precision=prec_main, reliable_every=a.reliable_every,
                              reliable_delta=delta)
     t_main, res_main, x_main = time_solves(lat, op, src_dev, arg_main, nrhs, a.warmup, a.steps, keep=True)
     ms = 1e3 * float(np.mean(t_main))
@@ -492,7 +495,7 @@ def main():
         "dtype": "f32" if mixed is not None else "f64",
         "data": "synthetic (seeded SU(3) links exp(i eps H), Philox)",
         "config": {"workload": cfg["name"], "nrhs": nrhs, "stop_rsd": 1e-10,
-                   "reliable_delta": delta if mixed is not None else None, "reliable_every": 100,
+                   "reliable_delta": delta if mixed is not None else None, "reliable_every": a.reliable_every,
                    "precision": {0: "fp64", 1: "mixed_fp16", 2: "mixed_bf16links", 3: "mixed_fp32"}[prec_main],
                    "storage": {0: "f64 spinors and links",
                                1: "f16 spinors (per-site norms) + f16 links, f32 CG vectors, f64 reliable updates",

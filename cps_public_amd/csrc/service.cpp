@@ -118,7 +118,9 @@ struct Engine {
     const long long V = (long long)ge.L[0] * ge.L[1] * ge.L[2] * ge.L[3];
     const long long per_rhs = (op.kind == CPS_HIP_OP_DWF || op.kind == CPS_HIP_OP_MOBIUS) ? V / 2 * ge.ls : V / 2;
     const long long vec = per_rhs * 192;
-    long long b = 8 * vec * S + vec + 2 * V * 72 * 8 + V * 48 + 64 * 1024 * 1024;
+    // FP64 fields x, b, r, Ap, t1, t2, t3 (+ p for FP64 CG), the CPS-order
+    // staging vector, links and neighbour table
+    long long b = (is_mixed(prec) ? 7 : 8) * vec * S + vec + 2 * V * 72 * 8 + V * 48 + 64 * 1024 * 1024;
     if (is_mixed(prec)) b += S * per_rhs * (96 + 5 * 96) + 2 * V * 72 * 4;   // sloppy fields (at most FP32), links
     return b;
   }
@@ -155,7 +157,7 @@ struct Engine {
     x = &L.field(S_X, STORE_D, n);
     b = &L.field(S_SRC, STORE_D, n);
     r = &L.field(S_R, STORE_D, n);
-    p = &L.field(S_P, STORE_D, n);
+    if (!is_mixed(prec)) p = &L.field(S_P, STORE_D, n);   // mixed: the direction is sloppy (ps)
     ap = &L.field(S_AP, STORE_D, n);
     t1 = &L.field(S_T1, STORE_D, n);
     t2 = &L.field(S_T2, STORE_D, n);
@@ -190,7 +192,8 @@ struct Engine {
     if ((rc = L.io.ensure((size_t)o.per_rhs * 24 * sizeof(double)))) return rc;
     // empty slots hold zeros and are inactive (and converged, so that a
     // reliable update leaves them alone)
-    for (Field* f : {x, b, r, p, ap, t1, t2, t3}) CPSHIP_CHECK(blas_zero(STORE_D, f->ref(), o.per_rhs, S, bsb, L.stream));
+    for (Field* f : {x, b, r, p, ap, t1, t2, t3})
+      if (f) CPSHIP_CHECK(blas_zero(STORE_D, f->ref(), o.per_rhs, S, bsb, L.stream));
     if (mixed)
       for (Field* f : {xs, rs, ps, aps, t1s, t2s})
         CPSHIP_CHECK(blas_zero(f->storage, f->ref(), o.per_rhs, S, bsb, L.stream));
@@ -225,11 +228,11 @@ struct Engine {
     const double* h = rq->src + (size_t)k * o.per_rhs * 24;
     CPSHIP_CHECK(hipMemcpyAsync(L.io.p, h, (size_t)o.per_rhs * 24 * sizeof(double), hipMemcpyHostToDevice, L.stream));
     const FieldRef bs = rhs_view(b->ref(), o.per_rhs, s), xs0 = rhs_view(x->ref(), o.per_rhs, s);
-    const FieldRef rs0 = rhs_view(r->ref(), o.per_rhs, s), ps0 = rhs_view(p->ref(), o.per_rhs, s);
+    const FieldRef rs0 = rhs_view(r->ref(), o.per_rhs, s);
     CPSHIP_CHECK(pack_spinor(STORE_D, (const double*)L.io.p, bs, o.map, L.g.Vcb, L.ls, 1, o.per_rhs, L.stream));
     CPSHIP_CHECK(blas_zero(STORE_D, xs0, o.per_rhs, 1, bsb, L.stream));
     CPSHIP_CHECK(convert_spinor(STORE_D, bs, STORE_D, rs0, o.per_rhs, L.stream));
-    CPSHIP_CHECK(convert_spinor(STORE_D, bs, STORE_D, ps0, o.per_rhs, L.stream));
+    if (p) CPSHIP_CHECK(convert_spinor(STORE_D, bs, STORE_D, rhs_view(p->ref(), o.per_rhs, s), o.per_rhs, L.stream));
     if (slot_mass) {
       const double kap = 1.0 / (2.0 * (rq->op.mass + 4.0));
       const double k2v = kap * kap, wv = -2.0 * rq->op.mass * op.a5_inv;

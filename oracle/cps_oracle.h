@@ -35,6 +35,8 @@ extern "C" {
 /* thread count of the OpenMP site loops; results do not depend on it */
 void ora_set_threads(int n);
 int ora_get_threads(void);
+/* ora_invcg prints |r|^2 to stderr every `every` iterations (0: silent) */
+void ora_set_progress(int every);
 
 void ora_gauge_canon_to_wilson(double *dst, const double *src, const int L[4]);
 void ora_gauge_wilson_to_canon(double *dst, const double *src, const int L[4]);

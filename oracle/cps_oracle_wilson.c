@@ -10,6 +10,7 @@
 
 #include <math.h>
 #include <omp.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -21,6 +22,11 @@ static int vol4(const int L[4]) { return L[0] * L[1] * L[2] * L[3]; }
 /* Thread count of the OpenMP site loops (results do not depend on it). */
 void ora_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 int ora_get_threads(void) { return omp_get_max_threads(); }
+
+/* ora_invcg prints |r|^2 to stderr every `every` iterations (0: silent), so
+ * that a long CPU solve reports its own progress. */
+static int progress_every = 0;
+void ora_set_progress(int every) { progress_every = every > 0 ? every : 0; }
 
 /* ======================================================================
  * Layout.  Follows convert_func.C:318-415 (gauge CANONICAL->WILSON,
@@ -567,6 +573,10 @@ int ora_invcg(double *sol, const double *src, size_t f_size_cb, double stop_rsd,
     ora_ftimesv1plusv2(res, -a, mmp, res, f_size_cb);
     res_norm_sq_cur = ora_dot(res, res, f_size_cb);
     if (res_hist) res_hist[itr - 1] = res_norm_sq_cur;
+    if (progress_every && itr % progress_every == 0) {
+      fprintf(stderr, "oracle CG iteration %d: |r|^2 %.3e (stop at %.3e)\n", itr, res_norm_sq_cur, stp_cnd);
+      fflush(stderr);
+    }
     if (res_norm_sq_cur <= stp_cnd) break;
     b = res_norm_sq_cur / res_norm_sq_prv;
     ora_ftimesv1plusv2(dir, b, dir, res, f_size_cb);

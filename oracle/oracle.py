@@ -53,6 +53,7 @@ def _declare(l):
     vp = ctypes.c_void_p
     sig = {
         "ora_set_threads": [ctypes.c_int],
+        "ora_set_progress": [ctypes.c_int],
         "ora_get_threads": [],
         "ora_gauge_canon_to_wilson": [vp, vp, vp],
         "ora_gauge_wilson_to_canon": [vp, vp, vp],
@@ -189,6 +190,11 @@ def set_threads(n):
 
 def get_threads():
     return int(lib().ora_get_threads())
+
+
+def set_progress(every):
+    """InvCg prints |r|^2 to stderr every `every` iterations (0: silent)."""
+    lib().ora_set_progress(int(every))
 
 
 # ---- vector BLAS ---------------------------------------------------------
