@@ -111,6 +111,13 @@ struct Engine {
   long long bytes = 0;
   clk::time_point idle_since = clk::now();
   std::unique_ptr<IterGraph> graph;
+  // the engine's last chunk of iterations is on the GPU until `done` fires
+  hipEvent_t done = nullptr;
+  bool in_flight = false;
+
+  ~Engine() {
+    if (done) (void)hipEventDestroy(done);
+  }
 
   static bool is_mixed(int prec) { return prec != CPS_HIP_PREC_DOUBLE; }
 
@@ -211,6 +218,7 @@ struct Engine {
     updates.assign(S, 0);
     bytes = L.device_bytes();
     graph.reset(new IterGraph(true));
+    CPSHIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     return 0;
   }
 
@@ -431,14 +439,16 @@ struct Service {
   int make_engine(Request* rq, Engine** out) {
     *out = nullptr;
     const GaugeEntry& ge = *gauges[rq->gauge_id];
-    const long long need = Engine::footprint(ge, rq->op, max_slots, rq->precision);
+    const int slots = max_slots;
+    const long long need = Engine::footprint(ge, rq->op, slots, rq->precision);
     while (used_bytes() + need > budget) {
-      auto idle = std::find_if(engines.begin(), engines.end(), [](const std::unique_ptr<Engine>& e) { return !e->busy(); });
+      auto idle = std::find_if(engines.begin(), engines.end(),
+                               [](const std::unique_ptr<Engine>& e) { return !e->busy() && !e->in_flight; });
       if (idle == engines.end()) return 0;   // wait: no room now
       engines.erase(idle);
     }
     std::unique_ptr<Engine> e(new Engine);
-    int rc = e->setup(ge, rq->gauge_id, *rq, max_slots);
+    int rc = e->setup(ge, rq->gauge_id, *rq, slots);
     if (rc) return rc;
     *out = e.get();
     engines.push_back(std::move(e));
@@ -469,10 +479,32 @@ struct Service {
           incoming.pop_front();
         }
       }
-      // admission: FIFO over waiting requests, one rhs per free slot
+      // Engines run independently: an engine whose last chunk finished (its
+      // event fired) is harvested, takes waiting rhs into its free slots and
+      // gets its next chunk; an engine still on the GPU is left alone, so a
+      // small lattice's engine cycles at its own pace beside a large one.
+      bool progress = false;
+      for (auto& e : engines) {
+        if (!e->in_flight) continue;
+        const hipError_t q = hipEventQuery(e->done);
+        if (q == hipErrorNotReady) continue;
+        (void)hipGetLastError();
+        e->in_flight = false;
+        progress = true;
+        std::vector<Request*> touched;
+        if (int rc = e->harvest(touched)) fatal = rc;
+        for (Request* rq : touched)
+          if (rq->done_rhs == rq->nrhs && !rq->finished) finish(rq);
+      }
+      // admission: FIFO over waiting requests, one rhs per free slot of an
+      // engine that is not on the GPU
       for (auto it = pending.begin(); it != pending.end();) {
         Request* rq = *it;
         Engine* e = find_engine(rq);
+        if (e && e->in_flight) {
+          ++it;
+          continue;
+        }
         int rc = e ? 0 : make_engine(rq, &e);
         if (!rc && !e && engines.empty()) rc = CPS_HIP_ERR_NOMEM;   // can never fit the budget
         while (!rc && e && rq->next_rhs < rq->nrhs && e->free_slots() > 0) {
@@ -492,27 +524,23 @@ struct Service {
         if (rq->next_rhs >= rq->nrhs) it = pending.erase(it);
         else ++it;
       }
-      // every busy engine advances one chunk; the launches of all engines are
-      // queued before any stream is synchronised, so their work overlaps
+      // every busy engine that is not on the GPU gets its next chunk
       long long stepped = 0;
       for (auto& e : engines)
-        if (e->busy()) {
+        if (e->busy() && !e->in_flight) {
           int ran = 0;
           if (int rc = e->step(chunk, &ran)) fatal = rc;
+          if (hipEventRecord(e->done, e->L.stream) != hipSuccess) fatal = CPS_HIP_ERR_RUNTIME;
+          e->in_flight = true;
           stepped += ran;
+          progress = true;
         }
-      for (auto& e : engines) {
-        if (!e->busy()) continue;
-        std::vector<Request*> touched;
-        if (int rc = e->harvest(touched)) fatal = rc;
-        for (Request* rq : touched)
-          if (rq->done_rhs == rq->nrhs && !rq->finished) finish(rq);
-      }
+      if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(50));
       // release engines idle for a while
       const auto now = clk::now();
       engines.erase(std::remove_if(engines.begin(), engines.end(),
                                    [&](const std::unique_ptr<Engine>& e) {
-                                     return !e->busy() &&
+                                     return !e->busy() && !e->in_flight &&
                                             std::chrono::duration<double>(now - e->idle_since).count() > 2.0;
                                    }),
                     engines.end());
