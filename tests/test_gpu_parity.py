@@ -972,7 +972,15 @@ def test_sitewave_mapping_bit_identical(oracle, monkeypatch, prec):
 def test_rhsgroup_mapping_bit_identical(monkeypatch, case):
     """Operators whose hops run with several rhs of one site per wavefront
     (HOP_RHSGROUP; clover with two kappas, rhs r on kappa r // 2) give the
-    same bits as the element mapping, FP64 and half precision."""
+    same bits as the element mapping in FP64 (outputs and |M x|^2).  The
+    half-precision kernels of the two mappings are separate compilations of
+    the same source: the compiler may pair the FP32 products of a sum
+    differently in its FMA contraction (the LLVM IR of the two instances has
+    the same operations in the same order), so a few FP32 results differ in
+    the last bit and, where that crosses an FP16 rounding boundary, a stored
+    element by one FP16 step (237 of 3.1M elements at 16^4 on the clover
+    operator; none at this size).  Half precision is therefore compared to
+    1e-6 (outputs) and 1e-10 (|M x|^2) rather than bit for bit."""
     L = [4, 4, 4, 8]
     u = wilson_gauge(L, 0.2, 250)
     ls = 8 if case == "mobius" else 1
@@ -992,10 +1000,13 @@ def test_rhsgroup_mapping_bit_identical(monkeypatch, case):
         else:
             op = C.make_op(C.OP_WILSON, mass=0.1)
         res[mode] = (lat.matpc(op, src, 0, nrhs=nrhs), lat.matpc(op, src, 1, nrhs=nrhs),
-                     *lat.mdagm_prec(op, src, C.PREC_MIXED_HALF, nrhs=nrhs), *lat.mdagm(op, src, nrhs=nrhs))
+                     *lat.mdagm(op, src, nrhs=nrhs), *lat.mdagm_prec(op, src, C.PREC_MIXED_HALF, nrhs=nrhs))
         lat.close()
-    for a, b_ in zip(res["element"], res["rhsgroup"]):
+    for a, b_ in zip(res["element"][:4], res["rhsgroup"][:4]):
         assert np.array_equal(a, b_)
+    (ha, da), (hb, db) = res["element"][4:], res["rhsgroup"][4:]
+    assert rel(hb, ha) < 1e-6
+    assert np.max(np.abs(db - da) / da) < 1e-10
 
 
 @pytest.mark.parametrize("paired", ["1", "0"])
@@ -1036,12 +1047,12 @@ def test_mixed_cg_mapping_consistent(monkeypatch, case):
     """A mixed-precision CG with the 4-D operators' hops free to use the
     element or the rhs-group mapping (the launch with the |M p|^2 chunk
     partials included) takes the same iterations to the same solution.  The
-    operators and the partial sums are bit-identical between the mappings
-    (test_rhsgroup_mapping_bit_identical); the clover solve with two kappa
-    sets still takes a different rounding path for the second set (measured
-    1.8e-12 apart; the same with per-block partials, where the launch with
-    partials never used the rhs-group mapping), so the solutions are
-    compared to 1e-9, well inside what a 1e-10 residual bound allows."""
+    partial sums use the same tree in both mappings, but the half-precision
+    kernels may round a few FP32 results differently
+    (test_rhsgroup_mapping_bit_identical): the two-kappa clover solve first
+    differs at iteration 7 (|M p|^2 of one rhs, 3.5e-10 relative) and ends
+    1.8e-12 apart, so the solutions are compared to 1e-9, well inside what a
+    1e-10 residual bound allows."""
     L = [4, 4, 4, 8]
     u = wilson_gauge(L, 0.2, 260)
     nrhs = 4
