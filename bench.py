@@ -171,6 +171,14 @@ def cpu_baseline(cfg, u_w, src, gpu_iters, full=False, legs=None):
     GPU FP64 iteration count (a projection, labelled as such)."""
     from oracle import oracle as O
     O.build()
+    O.use_timing_build(True)
+    try:
+        return _cpu_baseline(O, cfg, u_w, src, gpu_iters, full, legs)
+    finally:
+        O.use_timing_build(False)
+
+
+def _cpu_baseline(O, cfg, u_w, src, gpu_iters, full, legs):
     L, ls = cfg["L"], cfg["ls"]
     per = 24 * (int(np.prod(L)) // 2) * ls
     s0 = np.ascontiguousarray(src[:per])
@@ -182,7 +190,7 @@ def cpu_baseline(cfg, u_w, src, gpu_iters, full=False, legs=None):
         O.set_threads(nthr)
         clov = (kappa, O.clover_mat_chkb(u_w, 1, 0, kappa, cfg["csw"], L),
                 O.clover_mat_chkb(u_w, 0, 1, kappa, cfg["csw"], L))
-    out = {"unit": "s/RHS", "kind": "port", "host": host_info()}
+    out = {"unit": "s/RHS", "kind": "port", "host": host_info(), "build": O.TIMING_BUILD_FLAGS}
     if full:
         todo = legs if legs else ([nthr, 1] if nthr > 1 else [1])
         nthr = todo[0]

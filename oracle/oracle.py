@@ -16,22 +16,37 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
-_lib = None
+# the same sources built -O3 for x86-64-v3 with FP contraction: used only to
+# time the CPU baseline (results differ from liboracle.so by rounding)
+_FAST_PATH = os.path.join(_HERE, "liboracle_fast.so")
+TIMING_BUILD_FLAGS = "gcc -O3 -march=x86-64-v3 -fopenmp (FP contraction on)"
+_libs = {}
+_timing = False
 
 
 def build(force=False):
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"])
+    for path in (_LIB_PATH, _FAST_PATH):
+        if force or not os.path.exists(path):
+            subprocess.check_call(["make", "-s", "-C", _HERE] + (["-B"] if force else []) + [os.path.basename(path)])
+
+
+def use_timing_build(on=True):
+    """Route every following call to liboracle_fast.so (on) or back to the
+    checker build liboracle.so (off)."""
+    global _timing
+    _timing = bool(on)
 
 
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+    path = _FAST_PATH if _timing else _LIB_PATH
+    l = _libs.get(path)
+    if l is None:
+        if not os.path.exists(path):
             build()
-        _lib = ctypes.CDLL(_LIB_PATH)
-        _declare(_lib)
-    return _lib
+        l = ctypes.CDLL(path)
+        _declare(l)
+        _libs[path] = l
+    return l
 
 
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -623,3 +623,27 @@ def test_clover_preconditioned_ops(oracle, clover_case):
     A0 = oracle.clover_mat_chkb(uw, 1, 0, kappa, 0.0, L)
     A0i = oracle.clover_mat_chkb(uw, 0, 1, kappa, 0.0, L)
     assert relerr(oracle.clover_matpc(uw, A0, A0i, r_in, kappa, 0, L), oracle.wilson_m(uw, r_in, kappa, L)) < TOL
+
+
+def test_timing_build_is_the_same_oracle(oracle, gauge):
+    """liboracle_fast.so (the -O3 x86-64-v3 build with FP contraction that
+    times the CPU baseline) computes what the checker build computes: DWF
+    M^dag M and the Wilson CG agree to rounding, with the same iterations."""
+    L, ls = [4, 4, 4, 8], 8
+    u, _ = gauge(L, 0.3, 41)
+    uw = wilson_order_gauge(oracle, u, L)
+    d = oracle.dwf_struct(L, ls, 1.8)
+    rng = np.random.default_rng(42)
+    s5 = rng.standard_normal(oracle.dwf_half(d))
+    s4 = rng.standard_normal(24 * int(np.prod(L)) // 2)
+    got = {}
+    for fast in (False, True):
+        oracle.use_timing_build(fast)
+        try:
+            got[fast] = (oracle.dwf_mdagm(uw, s5, 0.05, d), oracle.wilson_invcg(uw, s4, 0.1, 1e-10, 500, L))
+        finally:
+            oracle.use_timing_build(False)
+    (m0, dp0), (x0, it0, _) = got[False]
+    (m1, dp1), (x1, it1, _) = got[True]
+    assert relerr(m1, m0) < 1e-14 and abs(dp1 - dp0) < 1e-13 * dp0
+    assert abs(it1 - it0) <= 1 and relerr(x1, x0) < 1e-9
