@@ -12,7 +12,8 @@
 // that arrive while others are on the GPU join as slots free up.  Engines of
 // different lattices or operators run concurrently on their own streams; a
 // new engine is created only while the HBM budget allows (admission by
-// footprint), and idle engines are released.
+// footprint), and engines idle for 30 s are released (sooner when the
+// budget needs their memory).
 //
 // A right-hand side's arithmetic never depends on the other slots: the hop
 // and BLAS kernels are element-wise, and every reduction sums one slot's
@@ -400,6 +401,9 @@ struct Engine {
 
 }  // namespace
 
+// an engine with no slot in use for this long is released
+constexpr double kIdleReleaseSeconds = 30.0;
+
 struct Service {
   long long budget = 0;
   int max_slots = 12;
@@ -536,12 +540,15 @@ struct Service {
           progress = true;
         }
       if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(50));
-      // release engines idle for a while
+      // release engines idle for a while (freeing device memory waits for
+      // the whole device, so an engine that may soon be needed again is kept;
+      // make_engine also releases idle engines when the budget requires it)
       const auto now = clk::now();
       engines.erase(std::remove_if(engines.begin(), engines.end(),
                                    [&](const std::unique_ptr<Engine>& e) {
                                      return !e->busy() && !e->in_flight &&
-                                            std::chrono::duration<double>(now - e->idle_since).count() > 2.0;
+                                            std::chrono::duration<double>(now - e->idle_since).count() >
+                                                kIdleReleaseSeconds;
                                    }),
                     engines.end());
       {

@@ -327,7 +327,8 @@ int cps_hip_bench_copy(cps_hip_lattice *lat, long long n_elem, int reps, double 
  * waiting rhs while the others iterate.  Wilson and DWF engines take a mass
  * per slot.  Engines run concurrently; an engine is created only while
  * the estimated device footprint fits hbm_budget_bytes (idle engines are
- * released first).  submit() returns at once; src and sol (host, CPS layout,
+ * released first; an engine idle for 30 s is released anyway).  submit()
+ * returns at once; src and sol (host, CPS layout,
  * nrhs vectors) must stay valid until wait() returns.  No counterpart in CPS. */
 typedef struct cps_hip_service cps_hip_service;
 typedef struct {
