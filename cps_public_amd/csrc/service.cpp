@@ -111,7 +111,8 @@ struct Engine {
   int since = 0;     // mixed: iterations since the last reliable update
   long long bytes = 0;
   clk::time_point idle_since = clk::now();
-  std::unique_ptr<IterGraph> graph;
+  // the iteration graph of each slot window [0, nw) (captured on first use)
+  std::vector<std::unique_ptr<IterGraph>> graphs;
   // the engine's last chunk of iterations is on the GPU until `done` fires
   hipEvent_t done = nullptr;
   bool in_flight = false;
@@ -218,7 +219,8 @@ struct Engine {
     rhs_of.assign(S, -1);
     updates.assign(S, 0);
     bytes = L.device_bytes();
-    graph.reset(new IterGraph(true));
+    graphs.clear();
+    graphs.resize(S + 1);
     CPSHIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     return 0;
   }
@@ -268,19 +270,36 @@ struct Engine {
     return 0;
   }
 
-  // up to `iters` CG iterations over all slots (inactive slots are frozen by
-  // the per-slot scalars), enqueued on the engine's stream; returns the
-  // number run (mixed: up to the next reliable update)
+  // the slot window [0, nw): every slot in use lies below nw (slots are
+  // filled lowest first), so the kernels leave out the empty slots above
+  int window() const {
+    int nw = 0;
+    for (int s = 0; s < S; s++)
+      if (owner[s]) nw = s + 1;
+    return nw;
+  }
+
+  IterGraph& graph_for(int nw) {
+    if (!graphs[nw]) graphs[nw].reset(new IterGraph(true));
+    return *graphs[nw];
+  }
+
+  // up to `iters` CG iterations over the slot window (inactive slots are
+  // frozen by the per-slot scalars), enqueued on the engine's stream;
+  // returns the number run (mixed: up to the next reliable update)
   int step(int iters, int* ran) {
+    const int nw = window();
+    *ran = 0;
+    if (nw == 0) return 0;
     if (!mixed) {
       *ran = iters;
-      return graph->run(L, iters, [&]() -> int {
-        int e = mdagm(PREC_D_D18, ap->ref(), p->ref(), t1->ref(), t2->ref(), part, S, 0);
+      return graph_for(nw).run(L, iters, [&]() -> int {
+        int e = mdagm(PREC_D_D18, ap->ref(), p->ref(), t1->ref(), t2->ref(), part, nw, 0);
         if (e) return e;
-        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, S, nullptr, RED_CG_ALPHA, cg.s, L.stream));
-        CPSHIP_CHECK(blas_cg_xr_d(x->ref(), p->ref(), r->ref(), ap->ref(), cg.s, o.per_rhs, S, bsb, part, L.stream));
-        CPSHIP_CHECK(reduce_rhs(part, bpr_blas, S, nullptr, RED_CG_BETA, cg.s, L.stream));
-        CPSHIP_CHECK(blas_xpay(STORE_D, p->ref(), r->ref(), cg.s.beta, cg.s.active, o.per_rhs, S, bsb, L.stream));
+        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, cg.s, L.stream));
+        CPSHIP_CHECK(blas_cg_xr_d(x->ref(), p->ref(), r->ref(), ap->ref(), cg.s, o.per_rhs, nw, bsb, part, L.stream));
+        CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, nullptr, RED_CG_BETA, cg.s, L.stream));
+        CPSHIP_CHECK(blas_xpay(STORE_D, p->ref(), r->ref(), cg.s.beta, cg.s.active, o.per_rhs, nw, bsb, L.stream));
         return 0;
       });
     }
@@ -289,15 +308,15 @@ struct Engine {
     since += n;
     // one sloppy iteration: p = r + beta p, A p, alpha, xs += alpha p and
     // r -= alpha A p with |r|^2, beta (the batch solver's unfused order)
-    return graph->run(L, n, [&]() -> int {
-      CPSHIP_CHECK(blas_xpay_mixed(sps, ps->ref(), svec, rs->ref(), cg.s.beta, cg.s.active, o.per_rhs, S, bsb,
+    return graph_for(nw).run(L, n, [&]() -> int {
+      CPSHIP_CHECK(blas_xpay_mixed(sps, ps->ref(), svec, rs->ref(), cg.s.beta, cg.s.active, o.per_rhs, nw, bsb,
                                    L.stream));
-      int e = mdagm(hprec, aps->ref(), ps->ref(), t1s->ref(), t2s->ref(), part, S, 0);
+      int e = mdagm(hprec, aps->ref(), ps->ref(), t1s->ref(), t2s->ref(), part, nw, 0);
       if (e) return e;
-      CPSHIP_CHECK(reduce_rhs(part, bpr_hop, S, nullptr, RED_CG_ALPHA, cg.s, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, cg.s, L.stream));
       CPSHIP_CHECK(blas_cg_xr_mixed(xvec, xs->ref(), sps, ps->ref(), svec, rs->ref(), sps, aps->ref(), cg.s, o.per_rhs,
-                                    S, bsb, part, L.stream));
-      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, S, nullptr, RED_CG_BETA, cg.s, L.stream));
+                                    nw, bsb, part, L.stream));
+      CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, nullptr, RED_CG_BETA, cg.s, L.stream));
       return 0;
     });
   }
@@ -307,18 +326,20 @@ struct Engine {
   // re-orthogonalised; slots whose FP64 residual meets the stopping rule
   // finish (mixed_after_update)
   int reliable_update() {
-    CPSHIP_CHECK(blas_axpy(xvec, xs->ref(), STORE_D, x->ref(), nullptr, 1.0, o.per_rhs, S, bsb, L.stream));
-    CPSHIP_CHECK(blas_zero(xvec, xs->ref(), o.per_rhs, S, bsb, L.stream));
-    int e = mdagm(PREC_D_D18, ap->ref(), x->ref(), t1->ref(), t2->ref(), nullptr, S, 0);
+    const int nw = window();
+    if (nw == 0) return 0;
+    CPSHIP_CHECK(blas_axpy(xvec, xs->ref(), STORE_D, x->ref(), nullptr, 1.0, o.per_rhs, nw, bsb, L.stream));
+    CPSHIP_CHECK(blas_zero(xvec, xs->ref(), o.per_rhs, nw, bsb, L.stream));
+    int e = mdagm(PREC_D_D18, ap->ref(), x->ref(), t1->ref(), t2->ref(), nullptr, nw, 0);
     if (e) return e;
-    CPSHIP_CHECK(blas_sub_norm_d(b->ref(), ap->ref(), r->ref(), o.per_rhs, S, bsb, part, L.stream));
-    CPSHIP_CHECK(reduce_rhs(part, bpr_blas, S, cg.rr_true, RED_PLAIN, cg.s, L.stream));
-    CPSHIP_CHECK(convert_spinor(STORE_D, r->ref(), svec, rs->ref(), o.per_rhs * S, L.stream));
-    CPSHIP_CHECK(blas_redot_mixed(svec, rs->ref(), sps, ps->ref(), o.per_rhs, S, bsb, part, L.stream));
-    CPSHIP_CHECK(reduce_rhs(part, bpr_blas, S, cg.s.alpha, RED_PLAIN, cg.s, L.stream));
+    CPSHIP_CHECK(blas_sub_norm_d(b->ref(), ap->ref(), r->ref(), o.per_rhs, nw, bsb, part, L.stream));
+    CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, cg.rr_true, RED_PLAIN, cg.s, L.stream));
+    CPSHIP_CHECK(convert_spinor(STORE_D, r->ref(), svec, rs->ref(), o.per_rhs * nw, L.stream));
+    CPSHIP_CHECK(blas_redot_mixed(svec, rs->ref(), sps, ps->ref(), o.per_rhs, nw, bsb, part, L.stream));
+    CPSHIP_CHECK(reduce_rhs(part, bpr_blas, nw, cg.s.alpha, RED_PLAIN, cg.s, L.stream));
     double* coef = (double*)coef_buf.p;
-    CPSHIP_CHECK(mixed_after_update(cg.s, cg.rr_true, cg.s.alpha, coef, S, L.stream));
-    CPSHIP_CHECK(blas_axpy(svec, rs->ref(), sps, ps->ref(), coef, 0.0, o.per_rhs, S, bsb, L.stream));
+    CPSHIP_CHECK(mixed_after_update(cg.s, cg.rr_true, cg.s.alpha, coef, nw, L.stream));
+    CPSHIP_CHECK(blas_axpy(svec, rs->ref(), sps, ps->ref(), coef, 0.0, o.per_rhs, nw, bsb, L.stream));
     for (int s = 0; s < S; s++)
       if (owner[s]) updates[s]++;
     since = 0;
