@@ -109,6 +109,11 @@ struct Engine {
   std::vector<Request*> owner;
   std::vector<int> rhs_of, updates;
   int since = 0;     // mixed: iterations since the last reliable update
+  // mixed: a slot stopped and the FP64 steps must run before it is harvested
+  bool need_update = false;
+  // mixed: slot s has had no sloppy iteration since the last reliable update
+  // (its solution and FP64 residual are current)
+  std::vector<char> fresh;
   long long bytes = 0;
   clk::time_point idle_since = clk::now();
   // the iteration graph of each slot window [0, nw) (captured on first use)
@@ -116,6 +121,9 @@ struct Engine {
   // the engine's last chunk of iterations is on the GPU until `done` fires
   hipEvent_t done = nullptr;
   bool in_flight = false;
+  // a host worker is uploading sources into the engine or copying its
+  // results out (the service thread leaves the engine alone meanwhile)
+  bool busy_host = false;
 
   ~Engine() {
     if (done) (void)hipEventDestroy(done);
@@ -218,6 +226,7 @@ struct Engine {
     owner.assign(S, nullptr);
     rhs_of.assign(S, -1);
     updates.assign(S, 0);
+    fresh.assign(S, 0);
     bytes = L.device_bytes();
     graphs.clear();
     graphs.resize(S + 1);
@@ -232,10 +241,20 @@ struct Engine {
   }
   bool busy() const { return free_slots() < S; }
 
+  // slot s assigned to rhs k of request rq (service thread; the device
+  // work is upload())
+  void claim(int s, Request* rq, int k) {
+    owner[s] = rq;
+    rhs_of[s] = k;
+    updates[s] = 0;
+    fresh[s] = 0;
+  }
+
   // rhs k of request rq into slot s: x = 0, r = p = src, |src|^2, stp, |r|^2
   // (mixed: the sloppy residual = src, xs = 0, p = 0 with beta = 0, and the
-  // slot's last FP64 residual = |src|^2); the slot's mass
-  int admit(int s, Request* rq, int k) {
+  // slot's last FP64 residual = |src|^2); the slot's mass.  Copies from the
+  // caller's memory, so it runs on a host worker.
+  int upload(int s, Request* rq, int k) {
     const double* h = rq->src + (size_t)k * o.per_rhs * 24;
     CPSHIP_CHECK(hipMemcpyAsync(L.io.p, h, (size_t)o.per_rhs * 24 * sizeof(double), hipMemcpyHostToDevice, L.stream));
     const FieldRef bs = rhs_view(b->ref(), o.per_rhs, s), xs0 = rhs_view(x->ref(), o.per_rhs, s);
@@ -257,6 +276,10 @@ struct Engine {
     CPSHIP_CHECK(blas_norm(STORE_D, rs0, o.per_rhs, 1, bsb, part, L.stream));
     CPSHIP_CHECK(reduce_rhs(part, bpr_blas, 1, mixed ? v.rr_upd : nullptr, RED_CG_INIT, v, L.stream));
     if (mixed) {
+      // a reliable update may come before the slot's first iteration (one
+      // requested for the other slots): its beta = |r|^2 / rr_prev must then
+      // be finite (p is zero, so the value itself does not matter)
+      CPSHIP_CHECK(hipMemcpyAsync(cg.s.rr_prev + s, cg.s.rr + s, sizeof(double), hipMemcpyDeviceToDevice, L.stream));
       CPSHIP_CHECK(convert_spinor(STORE_D, bs, svec, rhs_view(rs->ref(), o.per_rhs, s), o.per_rhs, L.stream));
       CPSHIP_CHECK(blas_zero(xvec, rhs_view(xs->ref(), o.per_rhs, s), o.per_rhs, 1, bsb, L.stream));
       CPSHIP_CHECK(blas_zero(sps, rhs_view(ps->ref(), o.per_rhs, s), o.per_rhs, 1, bsb, L.stream));
@@ -264,9 +287,6 @@ struct Engine {
       CPSHIP_CHECK(hipMemcpyAsync(cg.s.beta + s, &z, sizeof(double), hipMemcpyHostToDevice, L.stream));
       CPSHIP_CHECK(hipMemcpyAsync(cg.s.alpha + s, &z, sizeof(double), hipMemcpyHostToDevice, L.stream));
     }
-    owner[s] = rq;
-    rhs_of[s] = k;
-    updates[s] = 0;
     return 0;
   }
 
@@ -286,7 +306,9 @@ struct Engine {
 
   // up to `iters` CG iterations over the slot window (inactive slots are
   // frozen by the per-slot scalars), enqueued on the engine's stream;
-  // returns the number run (mixed: up to the next reliable update)
+  // returns the number run (mixed: up to the next reliable update).  Mixed:
+  // when a slot has stopped or `every` iterations have passed, the step is
+  // the reliable update alone, so the service thread never waits for it.
   int step(int iters, int* ran) {
     const int nw = window();
     *ran = 0;
@@ -303,9 +325,15 @@ struct Engine {
         return 0;
       });
     }
+    if (need_update || since >= every) {
+      need_update = false;
+      for (int s = 0; s < S; s++) fresh[s] = owner[s] != nullptr;
+      return reliable_update();
+    }
     const int n = std::max(1, std::min(iters, every - since));
     *ran = n;
     since += n;
+    std::fill(fresh.begin(), fresh.end(), 0);
     // one sloppy iteration: p = r + beta p, A p, alpha, xs += alpha p and
     // r -= alpha A p with |r|^2, beta (the batch solver's unfused order)
     return graph_for(nw).run(L, n, [&]() -> int {
@@ -346,34 +374,41 @@ struct Engine {
     return 0;
   }
 
-  // Slots whose CG stopped: FP64 true residual (inv_cg.C:403-417), solution
-  // copied to the caller, slot freed.  One synchronisation reads the slots'
-  // flags; the residuals, solutions and counters of every finished slot are
-  // then enqueued together and read back with one more.
-  int harvest(std::vector<Request*>& touched) {
+  // Slots whose CG stopped, in three parts.  scan() (service thread, when the
+  // engine's last step has finished and its stream is idle) reads the slots'
+  // flags with one synchronisation and lists the finished slots.  Mixed: a
+  // stopped slot is finished only once a reliable update has folded its
+  // sloppy solution into x and decided on its FP64 residual (which restarts
+  // it if the residual is not small enough); until then it asks for that
+  // update, which the next step runs.
+  int scan(std::vector<int>& done, std::vector<int>& st_done) {
     std::vector<int> act(S), st(S);
     CPSHIP_CHECK(hipMemcpyAsync(act.data(), cg.s.active, S * sizeof(int), hipMemcpyDeviceToHost, L.stream));
     CPSHIP_CHECK(hipMemcpyAsync(st.data(), cg.s.status, S * sizeof(int), hipMemcpyDeviceToHost, L.stream));
     CPSHIP_CHECK(hipStreamSynchronize(L.stream));
-    bool any = false;
-    for (int s = 0; s < S; s++)
-      if (owner[s] && !act[s]) any = true;
-    if (!any && !(mixed && since >= every)) return 0;
-    double* trn = cg.rr_true;
-    if (mixed) {
-      // a reliable update decides which stopped slots are finished (FP64
-      // residual) and restarts the others
-      if (int e = reliable_update()) return e;
-      CPSHIP_CHECK(hipMemcpyAsync(act.data(), cg.s.active, S * sizeof(int), hipMemcpyDeviceToHost, L.stream));
-      CPSHIP_CHECK(hipMemcpyAsync(st.data(), cg.s.status, S * sizeof(int), hipMemcpyDeviceToHost, L.stream));
-      CPSHIP_CHECK(hipStreamSynchronize(L.stream));
+    done.clear();
+    st_done.clear();
+    for (int s = 0; s < S; s++) {
+      if (!owner[s] || act[s]) continue;
+      if (mixed && (!fresh[s] || st[s] == CG_STATUS_NEED_UPDATE || st[s] == CG_STATUS_RUNNING)) {
+        need_update = true;
+        continue;
+      }
+      if (st[s] != CG_STATUS_NEED_UPDATE && st[s] != CG_STATUS_RUNNING) {
+        done.push_back(s);
+        st_done.push_back(st[s]);
+      }
     }
-    std::vector<int> done;
-    for (int s = 0; s < S; s++)
-      if (owner[s] && !act[s] && st[s] != CG_STATUS_NEED_UPDATE && st[s] != CG_STATUS_RUNNING) done.push_back(s);
-    if (done.empty()) return 0;
+    return 0;
+  }
+
+  // collect() (host worker): the FP64 true residual of the finished slots
+  // (inv_cg.C:403-417; mixed engines have it from the reliable update), their
+  // counters, and their solutions copied to the callers' memory, all enqueued
+  // together and read back with one synchronisation.
+  int collect(const std::vector<int>& done, std::vector<double>& tr, std::vector<double>& sn, std::vector<int>& it) {
+    double* trn = cg.rr_true;
     if (!mixed) {
-      // the FP64 true residual of the stopped slots only
       for (int s : done) {
         const FieldRef xw = rhs_view(x->ref(), o.per_rhs, s), aw = rhs_view(ap->ref(), o.per_rhs, s);
         int e = mdagm(PREC_D_D18, aw, xw, t1->ref(), t2->ref(), nullptr, 1, s);
@@ -383,8 +418,9 @@ struct Engine {
         CPSHIP_CHECK(reduce_rhs(part, bpr_blas, 1, trn + s, RED_PLAIN, scalars_view(cg.s, s), L.stream));
       }
     }
-    std::vector<double> tr(S), sn(S);
-    std::vector<int> it(S);
+    tr.assign(S, 0.0);
+    sn.assign(S, 0.0);
+    it.assign(S, 0);
     CPSHIP_CHECK(hipMemcpyAsync(tr.data(), trn, S * sizeof(double), hipMemcpyDeviceToHost, L.stream));
     CPSHIP_CHECK(hipMemcpyAsync(sn.data(), cg.s.src_norm, S * sizeof(double), hipMemcpyDeviceToHost, L.stream));
     CPSHIP_CHECK(hipMemcpyAsync(it.data(), cg.s.iters, S * sizeof(int), hipMemcpyDeviceToHost, L.stream));
@@ -400,30 +436,67 @@ struct Engine {
                                   (size_t)o.per_rhs * 24 * sizeof(double), hipMemcpyDeviceToHost, L.stream));
     }
     CPSHIP_CHECK(hipStreamSynchronize(L.stream));
-    for (int s : done) {
+    return 0;
+  }
+
+  // a host job failed with rc: its slots are freed and their requests fail
+  void fail(const std::vector<int>& slots, int rc, std::vector<Request*>& touched) {
+    for (int s : slots) {
       Request* rq = owner[s];
-      const int k = rhs_of[s];
-      cps_hip_cg_result& res = rq->res[k];
-      res.iterations = it[s] + 1;
-      res.status = st[s];
-      res.src_norm_sq = sn[s];
-      res.true_res = sn[s] > 0 ? std::sqrt(tr[s] / sn[s]) : 0.0;
-      res.reliable_updates = mixed ? updates[s] : 0;
-      if (st[s] == CG_STATUS_D_ZERO) rq->rc = CPS_HIP_ERR_D_ZERO;
+      if (!rq) continue;
+      rq->rc = rc;
       rq->done_rhs++;
       owner[s] = nullptr;
       rhs_of[s] = -1;
       touched.push_back(rq);
     }
     if (!busy()) idle_since = clk::now();
-    return 0;
   }
+
+  // release() (service thread): the results into the requests, slots freed
+  void release(const std::vector<int>& done, const std::vector<int>& st, const std::vector<double>& tr,
+               const std::vector<double>& sn, const std::vector<int>& it, std::vector<Request*>& touched) {
+    for (size_t i = 0; i < done.size(); i++) {
+      const int s = done[i];
+      Request* rq = owner[s];
+      const int k = rhs_of[s];
+      cps_hip_cg_result& res = rq->res[k];
+      res.iterations = it[s] + 1;
+      res.status = st[i];
+      res.src_norm_sq = sn[s];
+      res.true_res = sn[s] > 0 ? std::sqrt(tr[s] / sn[s]) : 0.0;
+      res.reliable_updates = mixed ? updates[s] : 0;
+      if (st[i] == CG_STATUS_D_ZERO) rq->rc = CPS_HIP_ERR_D_ZERO;
+      rq->done_rhs++;
+      owner[s] = nullptr;
+      rhs_of[s] = -1;
+      touched.push_back(rq);
+    }
+    if (!busy()) idle_since = clk::now();
+  }
+};
+
+// Host work of one engine done by a worker thread: sources uploaded into
+// claimed slots, or the results of finished slots copied to the callers
+struct HostJob {
+  Engine* e = nullptr;
+  bool harvest = false;
+  std::vector<int> slots;            // admit: the claimed slots; harvest: the finished ones
+  std::vector<Request*> rqs;         // admit: request and rhs of each slot
+  std::vector<int> ks;
+  std::vector<int> st;               // harvest: CG status of each finished slot
+  std::vector<double> tr, sn;        // harvest: per slot of the engine
+  std::vector<int> it;
+  int rc = 0;
 };
 
 }  // namespace
 
 // an engine with no slot in use for this long is released
 constexpr double kIdleReleaseSeconds = 30.0;
+// host worker threads: copies between the callers' memory and the device run
+// there, so a large lattice's transfers never hold up the other engines
+constexpr int kHostWorkers = 4;
 
 struct Service {
   long long budget = 0;
@@ -441,6 +514,50 @@ struct Service {
   int fatal = 0;
   std::thread th;
   cps_hip_service_stats stats;
+  int device = 0;   // the caller's device at creation; every service thread uses it
+  // host jobs: queued by the service thread, run by the workers, their
+  // results applied by the service thread
+  std::mutex job_mu;
+  std::condition_variable job_cv;
+  std::deque<std::unique_ptr<HostJob>> jobs, jobs_done;
+  std::vector<std::thread> workers;
+  bool workers_stop = false;
+  int jobs_out = 0;   // service thread only
+
+  void worker() {
+    (void)hipSetDevice(device);
+    for (;;) {
+      std::unique_ptr<HostJob> j;
+      {
+        std::unique_lock<std::mutex> lk(job_mu);
+        job_cv.wait(lk, [&] { return workers_stop || !jobs.empty(); });
+        if (jobs.empty()) return;
+        j = std::move(jobs.front());
+        jobs.pop_front();
+      }
+      Engine* e = j->e;
+      if (j->harvest) {
+        j->rc = e->collect(j->slots, j->tr, j->sn, j->it);
+      } else {
+        for (size_t i = 0; i < j->slots.size() && !j->rc; i++) j->rc = e->upload(j->slots[i], j->rqs[i], j->ks[i]);
+        if (!j->rc && hipStreamSynchronize(e->L.stream) != hipSuccess) j->rc = CPS_HIP_ERR_RUNTIME;
+      }
+      {
+        std::lock_guard<std::mutex> lk(job_mu);
+        jobs_done.push_back(std::move(j));
+      }
+    }
+  }
+
+  void dispatch(std::unique_ptr<HostJob> j) {
+    j->e->busy_host = true;
+    jobs_out++;
+    {
+      std::lock_guard<std::mutex> lk(job_mu);
+      jobs.push_back(std::move(j));
+    }
+    job_cv.notify_one();
+  }
 
   long long used_bytes() const {
     long long b = 0;
@@ -467,8 +584,9 @@ struct Service {
     const int slots = max_slots;
     const long long need = Engine::footprint(ge, rq->op, slots, rq->precision);
     while (used_bytes() + need > budget) {
-      auto idle = std::find_if(engines.begin(), engines.end(),
-                               [](const std::unique_ptr<Engine>& e) { return !e->busy() && !e->in_flight; });
+      auto idle = std::find_if(engines.begin(), engines.end(), [](const std::unique_ptr<Engine>& e) {
+        return !e->busy() && !e->in_flight && !e->busy_host;
+      });
       if (idle == engines.end()) return 0;   // wait: no room now
       engines.erase(idle);
     }
@@ -492,23 +610,49 @@ struct Service {
   }
 
   void loop() {
+    (void)hipSetDevice(device);
+    for (int w = 0; w < kHostWorkers; w++) workers.emplace_back([this] { worker(); });
     for (;;) {
       {
         std::unique_lock<std::mutex> lk(mu);
-        const bool idle = pending.empty() && std::none_of(engines.begin(), engines.end(),
-                                                         [](const std::unique_ptr<Engine>& e) { return e->busy(); });
+        const bool idle = pending.empty() && jobs_out == 0 &&
+                          std::none_of(engines.begin(), engines.end(),
+                                       [](const std::unique_ptr<Engine>& e) { return e->busy(); });
         if (idle) cv_work.wait(lk, [&] { return stop || !incoming.empty(); });
-        if (stop && incoming.empty() && idle) return;
+        if (stop && incoming.empty() && idle) break;
         while (!incoming.empty()) {
           pending.push_back(incoming.front());
           incoming.pop_front();
         }
       }
-      // Engines run independently: an engine whose last chunk finished (its
-      // event fired) is harvested, takes waiting rhs into its free slots and
-      // gets its next chunk; an engine still on the GPU is left alone, so a
-      // small lattice's engine cycles at its own pace beside a large one.
       bool progress = false;
+      // finished host jobs: uploads done (the engine may step), results
+      // copied out (slots freed, requests completed)
+      std::deque<std::unique_ptr<HostJob>> fin;
+      {
+        std::lock_guard<std::mutex> lk(job_mu);
+        fin.swap(jobs_done);
+      }
+      for (auto& j : fin) {
+        jobs_out--;
+        progress = true;
+        j->e->busy_host = false;
+        std::vector<Request*> touched;
+        if (j->rc) {
+          fatal = j->rc;
+          j->e->fail(j->slots, j->rc, touched);
+        } else if (j->harvest) {
+          j->e->release(j->slots, j->st, j->tr, j->sn, j->it, touched);
+        }
+        for (Request* rq : touched)
+          if (rq->done_rhs == rq->nrhs && !rq->finished) finish(rq);
+      }
+      // Engines run independently: an engine whose last chunk finished (its
+      // event fired) is scanned for finished slots (their results go to a
+      // host worker), takes waiting rhs into its free slots (uploaded by a
+      // worker) and gets its next chunk; an engine on the GPU or with a host
+      // job is left alone, so a small lattice's engine cycles at its own pace
+      // beside a large one and beside any engine's transfers.
       for (auto& e : engines) {
         if (!e->in_flight) continue;
         const hipError_t q = hipEventQuery(e->done);
@@ -516,17 +660,20 @@ struct Service {
         (void)hipGetLastError();
         e->in_flight = false;
         progress = true;
-        std::vector<Request*> touched;
-        if (int rc = e->harvest(touched)) fatal = rc;
-        for (Request* rq : touched)
-          if (rq->done_rhs == rq->nrhs && !rq->finished) finish(rq);
+        std::unique_ptr<HostJob> j(new HostJob);
+        j->e = e.get();
+        j->harvest = true;
+        if (int rc = e->scan(j->slots, j->st)) fatal = rc;
+        if (!j->slots.empty()) dispatch(std::move(j));
       }
       // admission: FIFO over waiting requests, one rhs per free slot of an
-      // engine that is not on the GPU
+      // engine that is neither on the GPU nor in a host job; the slots are
+      // claimed here and their sources uploaded by one job per engine
+      std::map<Engine*, std::unique_ptr<HostJob>> uploads;
       for (auto it = pending.begin(); it != pending.end();) {
         Request* rq = *it;
         Engine* e = find_engine(rq);
-        if (e && e->in_flight) {
+        if (e && (e->in_flight || e->busy_host)) {
           ++it;
           continue;
         }
@@ -535,8 +682,16 @@ struct Service {
         while (!rc && e && rq->next_rhs < rq->nrhs && e->free_slots() > 0) {
           int slot = 0;
           while (e->owner[slot]) slot++;
-          rc = e->admit(slot, rq, rq->next_rhs);
-          if (!rc) rq->next_rhs++;
+          std::unique_ptr<HostJob>& j = uploads[e];
+          if (!j) {
+            j.reset(new HostJob);
+            j->e = e;
+          }
+          e->claim(slot, rq, rq->next_rhs);
+          j->slots.push_back(slot);
+          j->rqs.push_back(rq);
+          j->ks.push_back(rq->next_rhs);
+          rq->next_rhs++;
         }
         if (rc) {
           // the request fails with rc: its remaining rhs are not solved, the
@@ -549,10 +704,15 @@ struct Service {
         if (rq->next_rhs >= rq->nrhs) it = pending.erase(it);
         else ++it;
       }
-      // every busy engine that is not on the GPU gets its next chunk
+      for (auto& kv : uploads) {
+        progress = true;
+        dispatch(std::move(kv.second));
+      }
+      // every busy engine that is neither on the GPU nor in a host job gets
+      // its next chunk
       long long stepped = 0;
       for (auto& e : engines)
-        if (e->busy() && !e->in_flight) {
+        if (e->busy() && !e->in_flight && !e->busy_host) {
           int ran = 0;
           if (int rc = e->step(chunk, &ran)) fatal = rc;
           if (hipEventRecord(e->done, e->L.stream) != hipSuccess) fatal = CPS_HIP_ERR_RUNTIME;
@@ -567,7 +727,7 @@ struct Service {
       const auto now = clk::now();
       engines.erase(std::remove_if(engines.begin(), engines.end(),
                                    [&](const std::unique_ptr<Engine>& e) {
-                                     return !e->busy() && !e->in_flight &&
+                                     return !e->busy() && !e->in_flight && !e->busy_host &&
                                             std::chrono::duration<double>(now - e->idle_since).count() >
                                                 kIdleReleaseSeconds;
                                    }),
@@ -580,6 +740,13 @@ struct Service {
         if (fatal && !stats.last_error) stats.last_error = fatal;
       }
     }
+    {
+      std::lock_guard<std::mutex> lk(job_mu);
+      workers_stop = true;
+    }
+    job_cv.notify_all();
+    for (auto& w : workers) w.join();
+    workers.clear();
   }
 };
 
@@ -603,6 +770,11 @@ int cps_hip_service_create(cps_hip_service** s, long long hbm_budget_bytes, int 
   if (!h) return CPS_HIP_ERR_NOMEM;
   h->S.budget = hbm_budget_bytes;
   h->S.max_slots = max_slots;
+  if (hipGetDevice(&h->S.device) != hipSuccess) {
+    (void)hipGetLastError();
+    delete h;
+    return CPS_HIP_ERR_RUNTIME;
+  }
   std::memset(&h->S.stats, 0, sizeof(h->S.stats));
   h->S.th = std::thread([h] { h->S.loop(); });
   *s = h;
