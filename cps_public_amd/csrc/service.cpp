@@ -10,10 +10,12 @@
 // (FP64 true residual, solution copied out) and refilled with the next
 // waiting right-hand side while the other slots keep iterating: requests
 // that arrive while others are on the GPU join as slots free up.  Engines of
-// different lattices or operators run concurrently on their own streams; a
-// new engine is created only while the HBM budget allows (admission by
-// footprint), and engines idle for 30 s are released (sooner when the
-// budget needs their memory).
+// different lattices or operators run concurrently on their own streams; the
+// service thread only schedules them, and the copies between the callers'
+// host memory and the device (sources in, solutions out) run on a few host
+// worker threads.  A new engine is created only while the HBM budget allows
+// (admission by footprint), and engines idle for 30 s are released (sooner
+// when the budget needs their memory).
 //
 // A right-hand side's arithmetic never depends on the other slots: the hop
 // and BLAS kernels are element-wise, and every reduction sums one slot's
@@ -600,6 +602,8 @@ struct Service {
     return 0;
   }
 
+  // rq complete: wait() may free it from here on, so this is the service
+  // thread's last access to it
   void finish(Request* rq) {
     std::lock_guard<std::mutex> lk(mu);
     rq->finished = true;
@@ -607,6 +611,15 @@ struct Service {
     stats.requests_done++;
     stats.rhs_done += rq->nrhs;
     cv_done.notify_all();
+  }
+
+  // the requests whose slots just finished (a request appears once per
+  // slot): each complete one is finished once, and none is read after that
+  void finish_touched(std::vector<Request*>& touched) {
+    std::sort(touched.begin(), touched.end());
+    touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+    for (Request* rq : touched)
+      if (rq->done_rhs == rq->nrhs && !rq->finished) finish(rq);
   }
 
   void loop() {
@@ -644,8 +657,7 @@ struct Service {
         } else if (j->harvest) {
           j->e->release(j->slots, j->st, j->tr, j->sn, j->it, touched);
         }
-        for (Request* rq : touched)
-          if (rq->done_rhs == rq->nrhs && !rq->finished) finish(rq);
+        finish_touched(touched);
       }
       // Engines run independently: an engine whose last chunk finished (its
       // event fired) is scanned for finished slots (their results go to a
@@ -699,10 +711,14 @@ struct Service {
           rq->rc = rc;
           rq->done_rhs += rq->nrhs - rq->next_rhs;
           rq->next_rhs = rq->nrhs;
-          if (rq->done_rhs == rq->nrhs) finish(rq);
         }
-        if (rq->next_rhs >= rq->nrhs) it = pending.erase(it);
-        else ++it;
+        if (rq->next_rhs >= rq->nrhs) {
+          it = pending.erase(it);
+          // last access: once finished, wait() may free the request
+          if (rc && rq->done_rhs == rq->nrhs) finish(rq);
+        } else {
+          ++it;
+        }
       }
       for (auto& kv : uploads) {
         progress = true;
