@@ -384,7 +384,11 @@ int hop_ex(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, con
 // variant on the real operands and keeps the fastest for the lattice
 // handle's lifetime.  Launches that write per-block reduction partials keep
 // the default block size, because the reduction reads one partial per block;
-// per-chunk partials (4-D operators) do not depend on the block or mapping.
+// per-chunk partials (4-D operators) are summed in an order that does not
+// depend on the block or mapping.  The candidates compute every element with
+// the same source arithmetic: FP64 results are the same bits whichever wins;
+// the half-precision kernels of different mappings are separate compilations
+// and may round a few FP32 results differently (FMA contraction).
 // CPS_HIP_TUNE=0 switches tuning off (default block, no variants).
 
 static std::mutex tuning_mu;
@@ -561,9 +565,9 @@ int tuned_launch(Lattice& L, HopParams& P) {
       // the same shape tuned for another rhs count (a CG's active-rhs window
       // shrinking): reuse that choice where it is valid for this count (for
       // launches with per-block partials it is the fixed-order choice, which
-      // does not depend on the count; chunk partials do not depend on the
-      // mapping, so a rhs-group choice that the count no longer allows falls
-      // back to element order with the same bits)
+      // does not depend on the count; chunk partials are summed in the same
+      // order in every mapping, so a rhs-group choice that the count no
+      // longer allows falls back to element order)
       for (const auto& kv : tune_cache) {
         HopTuneKey k2 = kv.first;
         k2.nrhs = key.nrhs;
