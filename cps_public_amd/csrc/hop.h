@@ -34,6 +34,10 @@ enum HopFlags {
                           // wavefront computes S sites x G rhs (x nb slots),
                           // G = 2 << ((flags >> 6) & 3), so the G lanes of a
                           // site share its link and clover loads
+  HOP_LINK_ROW = 256,     // half-precision spinors, nb = 16, element order:
+                          // the 16 lanes of a DPP row (one site's slots) load
+                          // one 4-byte word of each link each and exchange
+                          // them by row broadcasts
 };
 inline int hop_rhsgroup_G(int flags) { return (flags & HOP_RHSGROUP) ? (2 << ((flags >> 6) & 3)) : 1; }
 
@@ -180,6 +184,16 @@ inline bool rhsgroup_setup(HopParams& Q, int bs) {
   if (Q.nsites_out % ((bs / 64) * S)) return false;
   Q.rg_shift = __builtin_ctz((unsigned)G);
   return true;
+}
+
+// HOP_LINK_ROW: half-precision spinors, Wilson kind with nb = 16 (a DPP row
+// is one site's slots), element order (spatial allowed), blocks of whole
+// rows, and gauge planes addressable with 32-bit offsets
+inline bool link_row_setup(const HopParams& Q, int bs) {
+  const bool half = Q.prec == PREC_H_H12 || Q.prec == PREC_H_B12 || Q.prec == PREC_H_H18 || Q.prec == PREC_H_B18;
+  if (!half || Q.kind != KIND_WILSON || Q.nb != 16 || bs % 16) return false;
+  if (Q.flags & (HOP_SITEWAVE | HOP_LDS_LINKS | HOP_RHSGROUP)) return false;
+  return Q.gauge.plane * 16 * 3 < (int64_t(1) << 32);
 }
 
 // block size dividing n (a power of two, at most 256)

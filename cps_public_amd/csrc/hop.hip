@@ -31,12 +31,15 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
     shm = lds_link_bytes(Q.lds_sites);
   }
   if ((P.flags & HOP_RHSGROUP) && !rhsgroup_setup(Q, bs)) return hipErrorInvalidValue;
+  const bool row = (P.flags & HOP_LINK_ROW) != 0;
+  if (row && !link_row_setup(P, bs)) return hipErrorInvalidValue;
   // the fused fifth-dimension stage needs the 16 slots of a site in one row
   if (P.m5post && (!std::is_same<SIN, SpinH>::value || P.kind != KIND_WILSON || P.nb != 16 || P.X.data || bs % 16))
     return hipErrorInvalidValue;
   if (P.clov == CLOV_KS) {
     // a mass per rhs: element order, no variants
-    if (P.flags & (HOP_SPATIAL | HOP_SITEWAVE | HOP_LDS_LINKS | HOP_RHSGROUP) || P.m5post) return hipErrorInvalidValue;
+    if (P.flags & (HOP_SPATIAL | HOP_SITEWAVE | HOP_LDS_LINKS | HOP_RHSGROUP | HOP_LINK_ROW) || P.m5post)
+      return hipErrorInvalidValue;
     if (P.kind == KIND_DWF ? !P.kwall : !P.kscale) return hipErrorInvalidValue;
 #define CPSHIP_HOP_KS(DAG, KIND) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_KS, 0>), grid, block, 0, st, Q);
     if (P.kind == KIND_DWF) {
@@ -50,7 +53,13 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
 #define CPSHIP_HOP_LAUNCH(DAG, KIND, SW)                                                                      \
   if (nt) hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, true, CLOV_NONE, SW>), grid, block, shm, st, Q); \
   else hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, DAG, KIND, false, CLOV_NONE, SW>), grid, block, shm, st, Q);
-  if (lds) {
+  if (row) {
+    if constexpr (std::is_same<SIN, SpinH>::value) {
+      if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_WILSON, 4) } else { CPSHIP_HOP_LAUNCH(0, KIND_WILSON, 4) }
+    } else {
+      return hipErrorInvalidValue;
+    }
+  } else if (lds) {
     if constexpr (std::is_same<SIN, SpinH>::value) {
       if (P.kind == KIND_DWF) {
         if (P.dag) { CPSHIP_HOP_LAUNCH(1, KIND_DWF, 2) } else { CPSHIP_HOP_LAUNCH(0, KIND_DWF, 2) }

@@ -43,6 +43,7 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
     shm = lds_link_bytes(Q.lds_sites);
   }
   if ((P.flags & HOP_RHSGROUP) && !rhsgroup_setup(Q, bs)) return hipErrorInvalidValue;
+  if ((P.flags & HOP_LINK_ROW) && !link_row_setup(P, bs)) return hipErrorInvalidValue;
   if (P.m5post) return hipErrorInvalidValue;   // plain hops only
   if (P.s5_row && (P.clov != CLOV_S5X || P.nb != 16 || bs % 16)) return hipErrorInvalidValue;
   if (P.rupd && (!P.R.data || !P.ralpha || !P.ractive || (P.clov == CLOV_S5X ? !P.s5_row : P.clov != CLOV_X)))
@@ -53,6 +54,12 @@ static hipError_t launch_c(const HopParams& P, hipStream_t st) {
 #define CPSHIP_HOP_LAUNCH(DAG, CLOV)                                                              \
   if (P.flags & HOP_SITEWAVE) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 1) }                                \
   else if (P.flags & HOP_RHSGROUP) { CPSHIP_HOP_LAUNCH2(DAG, CLOV, 3) }                           \
+  else if (P.flags & HOP_LINK_ROW) {                                                              \
+    if constexpr (std::is_same<SIN, SpinH>::value && CLOV >= CLOV_S5X && CLOV <= CLOV_S5XRR) {    \
+      CPSHIP_HOP_LAUNCH2(DAG, CLOV, 4)                                                            \
+    }                                                                                             \
+    else return hipErrorInvalidValue;                                                             \
+  }                                                                                               \
   else if (lds) {                                                                                 \
     if constexpr (std::is_same<SIN, SpinH>::value && CLOV >= CLOV_S5X && CLOV <= CLOV_S5XRR) {                              \
       CPSHIP_HOP_LAUNCH2(DAG, CLOV, 2)                                                            \

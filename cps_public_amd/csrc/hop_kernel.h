@@ -729,6 +729,88 @@ __device__ __forceinline__ float dpp_f(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
 }
 
+// ---- HOP_LINK_ROW: links shared by a DPP row ------------------------------
+// With nb = 16 in element order the lanes 16r .. 16r+15 of a wavefront are
+// the 16 slots of one site, and every direction's link is the same for all
+// of them.  Lane j of the row loads 32-bit word min(j, NW - 1) of the link
+// (one 4-byte load and one register per link instead of 24 or 36 bytes), and
+// the row's lanes exchange the words by row broadcasts (v_mov_b32 with
+// row_newbcast); the words then take the path of a per-lane load.
+template <typename LK> struct LinkRowWords { static constexpr int n = 6; };   // 12-real: rows 0, 1
+template <> struct LinkRowWords<LinkH18> { static constexpr int n = 9; };
+template <> struct LinkRowWords<LinkB18> { static constexpr int n = 9; };
+
+// byte offset of lane j's word of link 0 and the stride between links: the
+// 16-byte plane holds words 0-3 (and the second 16-byte plane words 4-7 of
+// 18-real links); the last plane holds word 8 (18-real, 4 bytes per link)
+// or words 4, 5 (12-real, 8 bytes per link)
+template <typename LK>
+__device__ __forceinline__ void link_row_addr(const GaugeRef& g, int j, uint32_t& off, uint32_t& stride) {
+  constexpr int NW = LinkRowWords<LK>::n;
+  const int w = j < NW ? j : NW - 1;
+  const uint32_t pb = (uint32_t)(g.plane * 16);
+  if (w < 4) {
+    off = 4u * w;
+    stride = 16;
+  } else if (NW == 9 && w < 8) {
+    off = pb + 4u * (w - 4);
+    stride = 16;
+  } else if (NW == 9) {
+    off = 2 * pb;
+    stride = 4;
+  } else {
+    off = pb + 4u * (w - 4);
+    stride = 8;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t row_word(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, DPP_ROW_NEWBCAST + W, 0xF, 0xF, true);
+}
+
+// the link as a per-lane load would have delivered it, from the row's words
+template <typename LK>
+__device__ __forceinline__ LinkRaw<LK> link_from_row(uint32_t v) {
+  LinkRaw<LK> r;
+  r.a = make_uint4(row_word<0>(v), row_word<1>(v), row_word<2>(v), row_word<3>(v));
+  if constexpr (LinkRowWords<LK>::n == 9) {
+    r.b = make_uint4(row_word<4>(v), row_word<5>(v), row_word<6>(v), row_word<7>(v));
+    r.c = row_word<8>(v);
+  } else {
+    r.b = make_uint2(row_word<4>(v), row_word<5>(v));
+  }
+  return r;
+}
+
+// hop_batch with the links of HOP_LINK_ROW (half-precision spinors)
+template <int D0, int NB, typename LK, int DAG>
+__device__ __forceinline__ void hop_batch_row(const HopParams& P, const Nbr& N, int out_par, int site_cb,
+                                              int64_t in_base, int j, uint32_t loff, uint32_t lstride,
+                                              cplx<float> acc[12]) {
+  const int Vcb = P.geom.Vcb;
+  SpinRaw<SpinH> sr[NB];
+  uint32_t lw[NB];
+#pragma unroll
+  for (int k = 0; k < NB; k++) {
+    const int d = D0 + k;
+    raw_load<SpinH>(sr[k], P.in, in_base + (int64_t)N.idx[d] * P.nb + j);
+    const int mu = d & 3;
+    const uint32_t lidx = (d < 4) ? (uint32_t)((out_par * 4 + mu) * Vcb + site_cb)
+                                  : (uint32_t)(((1 - out_par) * 4 + mu) * Vcb + N.idx[d]);
+    lw[k] = ld_off<uint32_t>(P.gauge.data, 0, loff + lidx * lstride);
+  }
+#define CPSHIP_DIR(K)                                                                             \
+  if constexpr ((K) < NB) {                                                                       \
+    const float2 ph = link_phase(P, N.neg, D0 + (K));                                             \
+    uint32_t U[9];                                                                                \
+    link_words_f16<LK>(link_from_row<LK>(lw[K]), ph, U);                                          \
+    dir_compute_h2w<D0 + (K), DAG>(sr[K], U, acc);                                                \
+  }
+  CPSHIP_DIR(0) CPSHIP_DIR(1) CPSHIP_DIR(2) CPSHIP_DIR(3) CPSHIP_DIR(4) CPSHIP_DIR(5) CPSHIP_DIR(6) CPSHIP_DIR(7)
+#undef CPSHIP_DIR
+}
+
 // forward type: y_s = sum_{s' <= s} a^(s-s') x_s' (scan), then
 // y_s += fF_s y_15 (s < 15), y_15 *= inv_d
 __device__ __forceinline__ float m5_fwd(float x, bool last, const float (&ap)[4], float fF, float inv_d) {
@@ -815,7 +897,8 @@ __device__ __forceinline__ void m5_row(const HopParams& P, int s, cplx<float> (&
 // One output element: the hop, the fifth-dimension term (DWF), the epilogue
 // and the store; returns the element's |out|^2 when partials are requested
 // (|r|^2 of the updated residual for CLOV_S5XRR), summed in FP64.
-template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV, int LDSL>
+template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV, int LDSL,
+          bool ROWL = false>
 __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem& E, const Nbr& N, uint32_t* lds) {
   const int rhs = E.rhs, site = E.site, j = E.j;
   const int64_t per_rhs = (int64_t)P.nsites_out * P.nb;
@@ -853,8 +936,13 @@ __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem&
     }
   }
   if (valid) {
-    constexpr int NB = LDSL ? 0 : HopBatch<SIN>::value;
-    if constexpr (NB == 0) {
+    constexpr int NB = (LDSL || ROWL) ? 0 : HopBatch<SIN>::value;
+    if constexpr (ROWL && std::is_same<SIN, SpinH>::value) {
+      uint32_t loff, lstride;
+      link_row_addr<LK>(P.gauge, j, loff, lstride);
+      hop_batch_row<0, 4, LK, DAG>(P, N, out_par, site_cb, in_base, j, loff, lstride, acc);
+      hop_batch_row<4, 4, LK, DAG>(P, N, out_par, site_cb, in_base, j, loff, lstride, acc);
+    } else if constexpr (NB == 0) {
     } else if constexpr (NB == 8) {
       hop_batch<0, 8, SIN, LK, R, DAG>(P, N, out_par, site_cb, in_base, j, acc);
     } else if constexpr (NB == 4) {
@@ -1043,7 +1131,8 @@ __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem&
 // (HOP_SITEWAVE): one site per wavefront, so the site's neighbour entries
 // and links are read with scalar loads; SW = 2 (HOP_LDS_LINKS): element
 // order, links staged per block in LDS; SW = 3 (HOP_RHSGROUP): S sites x G
-// rhs per wavefront.
+// rhs per wavefront; SW = 4 (HOP_LINK_ROW): element order, links shared
+// across each 16-lane row.
 template <typename SIN, typename SOUT, typename LK, typename R, int DAG, int KIND, bool NT, int CLOV, int SW>
 __global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) {
   extern __shared__ uint4 hop_lds_area[];   // HOP_LDS_LINKS (SW = 2): the block's links
@@ -1052,7 +1141,7 @@ __global__ void __launch_bounds__(256, HopWaves<SIN>::value) k_hop(HopParams P) 
   const HopElem E = SW == 1 ? hop_decode_sitewave(P, hb, nsub)
                    : SW == 3 ? hop_decode_rhsgroup(P, hb) : hop_decode<KIND>(P, hb, nsub);
   const Nbr N = hop_nbr<KIND>(P, E);
-  const double nrm = hop_element<SIN, SOUT, LK, R, DAG, KIND, NT, CLOV, SW == 2>(
+  const double nrm = hop_element<SIN, SOUT, LK, R, DAG, KIND, NT, CLOV, SW == 2, SW == 4>(
       P, E, N, reinterpret_cast<uint32_t*>(hop_lds_area));
   if (P.partial) {
     if (!P.chunk_part) block_partial<double>(nrm, P.partial, E.pslot);

@@ -410,7 +410,7 @@ cps_hip_tuning tuning() {
   t.tune_report = env_int("CPS_HIP_TUNE_REPORT", t.tune_report);
   if (const char* m = getenv("CPS_HIP_HOP_MODE"))
     t.hop_mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3
-               : !strcmp(m, "rhsgroup") ? 4 : 0;
+               : !strcmp(m, "rhsgroup") ? 4 : !strcmp(m, "row") ? 5 : 0;
   t.rhs_window = env_int("CPS_HIP_RHS_WINDOW", t.rhs_window);
   t.lagged_x = env_int("CPS_HIP_LAGGED_X", t.lagged_x);
   t.paired_x = env_int("CPS_HIP_PAIRED_X", t.paired_x);
@@ -447,7 +447,7 @@ extern "C" int cps_hip_get_tuning(cps_hip_tuning* t) {
 }
 
 extern "C" int cps_hip_set_tuning(const cps_hip_tuning* t) {
-  if (!t || t->hop_mode < 0 || t->hop_mode > 4 || t->sloppy_vectors < 0 || t->sloppy_vectors > 3)
+  if (!t || t->hop_mode < 0 || t->hop_mode > 5 || t->sloppy_vectors < 0 || t->sloppy_vectors > 3)
     return CPS_HIP_ERR_ARG;
   std::lock_guard<std::mutex> lk(cpship::tuning_mu);
   cpship::tuning_fields = *t;
@@ -543,14 +543,15 @@ int tuned_launch(Lattice& L, HopParams& P) {
     CPSHIP_CHECK(launch_hop(P, L.stream));
     return 0;
   }
-  // hop_mode: the field, or CPS_HIP_HOP_MODE=element|sitewave|lds|rhsgroup
+  // hop_mode: the field, or CPS_HIP_HOP_MODE=element|sitewave|lds|rhsgroup|row
   int mode;
   {
     std::lock_guard<std::mutex> lk(tuning_mu);
     mode = tuning_fields.hop_mode;
   }
   if (const char* m = getenv("CPS_HIP_HOP_MODE"))
-    mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3 : !strcmp(m, "rhsgroup") ? 4 : 0;
+    mode = !strcmp(m, "sitewave") ? 2 : !strcmp(m, "element") ? 1 : !strcmp(m, "lds") ? 3 : !strcmp(m, "rhsgroup") ? 4
+         : !strcmp(m, "row") ? 5 : 0;
   HopTuneKey key{P.kind, P.prec, P.dag, P.nb, P.nsites_out, P.nrhs, P.partial != nullptr, P.X.data != nullptr,
                  P.clov + 16 * P.m5post + 32 * P.rupd, {L.g.L[0], L.g.L[1], L.g.L[2], L.g.L[3]}, mode};
   HopTuneVal chosen;
@@ -605,13 +606,28 @@ int tuned_launch(Lattice& L, HopParams& P) {
       CPSHIP_CHECK(hipEventCreate(&e1));
       float best_ms = 1e30f;
       // candidate flags: every combination of the first five variants, then
-      // the rhs-group mapping for G = 2..16 with and without streaming stores
+      // the rhs-group mapping for G = 2..16 with and without streaming
+      // stores, then the row-shared links with the order variants
       std::vector<int> cand;
       for (int f = 0; f < 32; f++) cand.push_back(f);
       for (int g = 0; g < 4; g++)
         for (int nt = 0; nt < 2; nt++) cand.push_back(HOP_RHSGROUP | (g << 6) | nt);
+      for (int f = 0; f < 8; f++) cand.push_back(HOP_LINK_ROW | f);
       for (int b : blocks)
         for (int f : cand) {
+          {
+            // element order with the links shared by each row: a candidate
+            // without mode restriction, the only one where possible with
+            // CPS_HIP_HOP_MODE=row, none with the other restrictions
+            HopParams Q = P;
+            Q.flags = HOP_LINK_ROW;
+            const bool row_ok = link_row_setup(Q, b);
+            if (f & HOP_LINK_ROW) {
+              if ((mode != 0 && mode != 5) || !row_ok) continue;
+            } else if (mode == 5 && row_ok) {
+              continue;
+            }
+          }
           if (f & HOP_RHSGROUP) {
             if (!rhsgroup_ok(P, b, f) || mode == 1 || mode == 2 || mode == 3) continue;
           } else if (mode == 4 && (rhsgroup_ok(P, b, HOP_RHSGROUP) || rhsgroup_ok(P, def_block, HOP_RHSGROUP))) {

@@ -1103,6 +1103,46 @@ def test_lds_links_mapping_bit_identical(monkeypatch, kind, ls, nrhs):
     assert np.abs(res["lds"][0]).max() > 0
 
 
+@pytest.mark.parametrize("links", [C.PREC_MIXED_HALF, C.PREC_MIXED_BF16])
+def test_link_row_mapping(oracle, monkeypatch, links):
+    """Half-precision Moebius operator (Ls 16) with each link loaded once per
+    16-lane row and shared by row broadcasts (HOP_LINK_ROW,
+    CPS_HIP_HOP_MODE=row): M^dag M and |M x|^2 agree with the element mapping
+    to FP32 rounding (a separate compilation of the same arithmetic) and with
+    the FP64 oracle to half-precision accuracy, for FP16 and BF16 links and 1
+    and 2 rhs (18-real links: with 16 slots per site the 12-real form is
+    never chosen); the mixed CG takes the same iterations (within one) to the
+    same solution."""
+    L, ls = [4, 4, 4, 8], 16
+    mass, m5, b, c = 0.05, 1.8, 1.5, 0.5
+    u = wilson_gauge(L, 0.2, 232)
+    op = C.make_op(C.OP_MOBIUS, mass=mass, m5=m5, b=b, c=c)
+    vcb = int(np.prod(L)) // 2
+    n = 24 * vcb * ls
+    arg = C.make_cg_arg(1e-10, 3000, precision=links, reliable_every=50)
+    res = {}
+    for nrhs in (2, 1):
+        src = np.concatenate([F.gaussian_fermion(vcb * ls, 233 + k) for k in range(nrhs)])
+        for mode in ("element", "row"):
+            monkeypatch.setenv("CPS_HIP_HOP_MODE", mode)
+            lat = C.Lattice(L, ls, (0, 0, 0, 1))
+            lat.import_gauge(u)
+            out, dp = lat.mdagm_prec(op, src, links, nrhs=nrhs)
+            x, r = lat.invcg(op, src, arg, nrhs=nrhs)
+            res[(nrhs, mode)] = (out, dp, x, [k.iterations for k in r], [k.true_res for k in r])
+            lat.close()
+        (oe, de, xe, ie, te), (orow, drow, xrow, irow, trow) = res[(nrhs, "element")], res[(nrhs, "row")]
+        assert np.abs(orow).max() > 0
+        assert rel(orow, oe) < 1e-6 and np.max(np.abs(drow - de) / de) < 1e-10
+        m = oracle.mobius_struct(L, ls, m5, b, c)
+        for k in range(nrhs):
+            ref = oracle.mobius_mdagm(u, src[k * n:(k + 1) * n], mass, m)[0]
+            assert rel(orow[k * n:(k + 1) * n], ref) < 2e-2
+        assert max(te + trow) <= 1e-10
+        assert all(abs(a - b_) <= 1 for a, b_ in zip(ie, irow))
+        assert rel(xrow, xe) < 1e-9
+
+
 # ---------------------------------------------------------------------------
 # CANONICAL order on the device: gauge import (Convert + BondCond) and
 # Lattice::FmatInv on CANONICAL vectors
