@@ -607,7 +607,8 @@ int tuned_launch(Lattice& L, HopParams& P) {
       float best_ms = 1e30f;
       // candidate flags: every combination of the first five variants, then
       // the rhs-group mapping for G = 2..16 with and without streaming
-      // stores, then the row-shared links with the order variants
+      // stores, then the row-shared links with the order variants (tried
+      // with CPS_HIP_HOP_MODE=row only)
       std::vector<int> cand;
       for (int f = 0; f < 32; f++) cand.push_back(f);
       for (int g = 0; g < 4; g++)
@@ -616,14 +617,16 @@ int tuned_launch(Lattice& L, HopParams& P) {
       for (int b : blocks)
         for (int f : cand) {
           {
-            // element order with the links shared by each row: a candidate
-            // without mode restriction, the only one where possible with
-            // CPS_HIP_HOP_MODE=row, none with the other restrictions
+            // element order with the links shared by each row: only with
+            // CPS_HIP_HOP_MODE=row (then the only candidate where possible);
+            // measured no faster than the element order at C3h (the M5^-1
+            // hop 4.47 against 3.63 ms, the s-term hop a tie), so the
+            // default search leaves it out
             HopParams Q = P;
             Q.flags = HOP_LINK_ROW;
             const bool row_ok = link_row_setup(Q, b);
             if (f & HOP_LINK_ROW) {
-              if ((mode != 0 && mode != 5) || !row_ok) continue;
+              if (mode != 5 || !row_ok) continue;
             } else if (mode == 5 && row_ok) {
               continue;
             }

@@ -110,7 +110,7 @@ typedef struct {
   int use_graph;        /* 1: CG iterations replayed as a captured hipGraph (ANDed with cps_hip_cg_arg.use_graph) */
   int tune;             /* 1: the hop autotuner times its candidates; 0: default launch */
   int tune_report;      /* 1: print the tuner's choices, 2: also every candidate's time */
-  int hop_mode;         /* restrict the hop's thread mapping: 0 any, 1 element, 2 sitewave, 3 lds, 4 rhsgroup, 5 row-shared links */
+  int hop_mode;         /* restrict the hop's thread mapping: 0 any, 1 element, 2 sitewave, 3 lds, 4 rhsgroup, 5 row-shared links (only reachable this way) */
   int rhs_window;       /* 1: kernels run on the contiguous range of rhs still iterating */
   int lagged_x;         /* mixed CG: solution step applied by the next direction update */
   int paired_x;         /* mixed CG: two solution steps per pass over x (FP16 directions) */
