@@ -43,8 +43,10 @@ HBM_PEAK_GBS = 8000.0
 CONFIGS = {
     "c1": dict(name="wilson_8^4_kappa0.12", L=[8, 8, 8, 8], ls=1, kind=C.OP_WILSON, eps=0.3,
                mass=1.0 / (2 * 0.12) - 4.0, nrhs=1, src="gauss", mixed=None, seed=101),
+    # reliable-update trigger 0.03: 172 iterations and 7 FP64 updates
+    # (0.1: 171 and 10, 0.01: 244 and 6)
     "c2": dict(name="shamir_dwf_16^3x32_ls16_mf0.01", L=[16, 16, 16, 32], ls=16, kind=C.OP_DWF, eps=0.2,
-               mass=0.01, m5=1.8, nrhs=1, src="gauss", mixed=C.PREC_MIXED_HALF, seed=202),
+               mass=0.01, m5=1.8, nrhs=1, src="gauss", mixed=C.PREC_MIXED_HALF, seed=202, reliable_delta=0.03),
     "c3": dict(name="mobius_dwf_24^3x64_ls16_b1.5_c0.5_mf0.005_12rhs", L=[24, 24, 24, 64], ls=16,
                kind=C.OP_MOBIUS, eps=0.2, mass=0.005, m5=1.8, b=1.5, c=0.5, nrhs=12, src="point",
                mixed=C.PREC_MIXED_BF16, seed=303),
