@@ -55,9 +55,11 @@ CONFIGS = {
     "c3h": dict(name="mobius_dwf_24^3x64_ls16_b1.5_c0.5_mf0.005_12rhs_fp16links", L=[24, 24, 24, 64], ls=16,
                 kind=C.OP_MOBIUS, eps=0.2, mass=0.005, m5=1.8, b=1.5, c=0.5, nrhs=12, src="point",
                 mixed=C.PREC_MIXED_HALF, seed=303, reliable_delta=0.01),
+    # reliable-update trigger 0.03: 86-101 iterations, 7-8 FP64 updates per
+    # rhs (0.01: 90-109 and 5-6; 0.1: 86-100 and 10-11)
     "c4": dict(name="clover_32^3x64_csw1.0_kappa0.1360-0.1375_48rhs", L=[32, 32, 32, 64], ls=1,
                kind=C.OP_CLOVER, eps=0.15, kappas=[0.1360, 0.1365, 0.1370, 0.1375], csw=1.0, nrhs=48,
-               src="point", mixed=C.PREC_MIXED_HALF, seed=404, reliable_delta=0.01),
+               src="point", mixed=C.PREC_MIXED_HALF, seed=404, reliable_delta=0.03),
 }
 
 
