@@ -715,6 +715,28 @@ def test_service_results_same_alone_and_in_mix(oracle):
     svc.close()
 
 
+def test_service_request_beyond_budget_fails_cleanly():
+    """Requests whose engine can never fit the HBM budget fail with
+    CPS_HIP_ERR_NOMEM through the admission error path (each request
+    finished once, none read after wait() has freed it), several of them
+    in a row, and the service shuts down normally afterwards."""
+    L = [4, 4, 4, 8]
+    u = wilson_gauge(L, 0.2, 175)
+    svc = C.Service(1 << 20, max_slots=2)   # room for the gauge field, not for an engine
+    g = svc.add_gauge(L, 1, u)
+    op = C.make_op(C.OP_WILSON, mass=0.1)
+    vcb = int(np.prod(L)) // 2
+    rids = [svc.submit(g, op, np.concatenate([F.gaussian_fermion(vcb, 176 + k) for k in range(nrhs)]), nrhs)
+            for k, nrhs in enumerate((1, 3, 2))]
+    for rid in rids:
+        with pytest.raises(C.CpsHipError) as err:
+            svc.wait(rid)
+        assert err.value.code == -5
+    st = svc.stats()
+    assert st["requests_done"] == 3 and st["engines_created"] == 0
+    svc.close()
+
+
 @pytest.mark.parametrize("prec", [C.PREC_MIXED_HALF, C.PREC_MIXED_SINGLE])
 def test_service_mixed_precision_shared_masses(oracle, prec):
     """Mixed-precision requests (sloppy CG with FP64 reliable updates per
