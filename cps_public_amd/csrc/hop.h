@@ -132,8 +132,12 @@ struct HopParams {
 // launcher; not values of HopParams::clov)
 // CLOV_KS: no clover term, a mass per rhs (kscale for the Wilson kind, kwall
 // for the DWF kind), element order only; a value of HopParams::clov
+// CLOV_NRR: no clover term, out = xcoef X + scale hop as for CLOV_NONE, and
+// the CG residual update (rupd) instead of storing out (half-precision DWF
+// hops; the kernel the launcher picks for CLOV_NONE with rupd, not a value
+// of HopParams::clov)
 enum { CLOV_NONE = 0, CLOV_POST = 1, CLOV_X = 2, CLOV_S5X = 3, CLOV_S5XR = 4, CLOV_S5XRR = 5, CLOV_XRR = 6,
-       CLOV_KS = 7 };
+       CLOV_KS = 7, CLOV_NRR = 8 };
 
 hipError_t launch_hop(const HopParams& P, hipStream_t st);
 hipError_t build_nbr(const Geom& g, uint4* nbr, uint32_t* nbr_neg, hipStream_t st);

@@ -36,6 +36,23 @@ static hipError_t launch_t(const HopParams& P, hipStream_t st) {
   // the fused fifth-dimension stage needs the 16 slots of a site in one row
   if (P.m5post && (!std::is_same<SIN, SpinH>::value || P.kind != KIND_WILSON || P.nb != 16 || P.X.data || bs % 16))
     return hipErrorInvalidValue;
+  if (P.rupd) {
+    // the CG residual update instead of the store (CLOV_NRR): half-precision
+    // DWF hops with X, element order, no variants
+    if (P.clov != CLOV_NONE || P.kind != KIND_DWF || !P.X.data || !P.R.data || !P.ralpha || !P.ractive || P.m5post ||
+        (P.flags & (HOP_SPATIAL | HOP_SITEWAVE | HOP_LDS_LINKS | HOP_RHSGROUP | HOP_LINK_ROW)))
+      return hipErrorInvalidValue;
+    if constexpr (std::is_same<SIN, SpinH>::value) {
+      if (P.dag) {
+        hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, 1, KIND_DWF, false, CLOV_NRR, 0>), grid, block, 0, st, Q);
+      } else {
+        hipLaunchKernelGGL((k_hop<SIN, SOUT, LK, R, 0, KIND_DWF, false, CLOV_NRR, 0>), grid, block, 0, st, Q);
+      }
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   if (P.clov == CLOV_KS) {
     // a mass per rhs: element order, no variants
     if (P.flags & (HOP_SPATIAL | HOP_SITEWAVE | HOP_LDS_LINKS | HOP_RHSGROUP | HOP_LINK_ROW) || P.m5post)

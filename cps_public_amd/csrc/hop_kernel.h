@@ -1060,7 +1060,7 @@ __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem&
           res[c_from_p + k].im += scp * pim;
         }
       }
-    } else if constexpr (CLOV != CLOV_NONE && CLOV != CLOV_KS) {
+    } else if constexpr (CLOV != CLOV_NONE && CLOV != CLOV_KS && CLOV != CLOV_NRR) {
       const int set = (P.rhs0 + rhs) / P.rhs_per_set;
       // (no branch here: a conditional load splits the block and the stencil
       // above it then spills)
@@ -1101,7 +1101,7 @@ __device__ __forceinline__ double hop_element(const HopParams& P, const HopElem&
         if (P.m5post) m5_row<DAG>(P, j, res);
       }
     }
-    if constexpr (CLOV == CLOV_S5XRR || CLOV == CLOV_XRR) {
+    if constexpr (CLOV == CLOV_S5XRR || CLOV == CLOV_XRR || CLOV == CLOV_NRR) {
       // the CG step r -= alpha (M^dag M p) and |r|^2, with M^dag M p in FP32
       if (P.ractive[rhs]) {
         const R a = R(P.ralpha[rhs]);

@@ -271,6 +271,11 @@ int clover_mdagm(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& ou
                  const FieldRef& t1, const FieldRef& t2, double* partial, int nrhs);
 int clover_mdag_rupdate(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& r, const FieldRef& t2,
                         const FieldRef& t1, const double* alpha, const int* active, double* partial, int nrhs);
+// DWF M^dag of t3 with the CG residual step in the epilogue of its second
+// hop: r -= alpha[rhs] (M^dag t3) for the rhs with active[rhs], partial =
+// |r|^2 (FP16 r and hops; M^dag M p is never stored)
+int dwf_mdag_rupdate(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& r, const FieldRef& t3,
+                     const FieldRef& t1, const double* alpha, const int* active, double* partial, int nrhs);
 int clover_build(Lattice& L, const double* kappas, int n_kappa, double csw);
 int clover_export(Lattice& L, int k, int cb, double* A);
 // MatInv source preparation (src = MatPcDag of the prepared odd source) and

@@ -700,6 +700,22 @@ int apply_m(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& out, co
   return hop(L, op, hprec, out, t, 1, dag, -k * k, in, 1.0, partial, nrhs);
 }
 
+// M^dag t3 = t3 - kappa^2 D^dag D^dag t3 (dwf_mdag.C:32-62) with r -= alpha
+// M^dag t3 and |r|^2 in the epilogue of the second hop (CLOV_NRR)
+int dwf_mdag_rupdate(Lattice& L, const cps_hip_op* op, int hprec, const FieldRef& r, const FieldRef& t3,
+                     const FieldRef& t1, const double* alpha, const int* active, double* partial, int nrhs) {
+  if (op->kind != CPS_HIP_OP_DWF || hprec == PREC_D_D18 || hprec == PREC_F_F12) return CPS_HIP_ERR_ARG;
+  const double k = op_kappa(op);
+  int rc = hop(L, op, hprec, t1, t3, 0, 1, 1.0, null_field(), 0.0, nullptr, nrhs);
+  if (rc) return rc;
+  HopClover h{};
+  h.mode = CLOV_NONE;
+  h.R = &r;
+  h.ralpha = alpha;
+  h.ractive = active;
+  return hop_ex(L, op, hprec, r, t1, 1, 1, -k * k, t3, 1.0, partial, nrhs, &h);
+}
+
 // apply_m with a mass per rhs (Wilson and DWF kinds): rhs r of the launch
 // uses kappa^2 = k2[rhs0 + r] (Wilson: the second hop's scale) or the wall
 // coefficient wall[rhs0 + r] = -2 mf / a5 (DWF: both hops; kappa_5 does not
@@ -1218,6 +1234,10 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   // 2.56 s.
   const bool clover_rih = lagged && op->kind == CPS_HIP_OP_CLOVER && sps == STORE_H && svec == STORE_H &&
                           tn.clover_r_in_hop != 0;
+  // DWF, FP16 directions and residual: the residual step in the epilogue of
+  // the second hop of M^dag (CPS_HIP_R_IN_HOP=0: a separate pass)
+  const bool dwf_rih = lagged && op->kind == CPS_HIP_OP_DWF && sps == STORE_H && svec == STORE_H &&
+                       tn.r_in_hop != 0;
   Field& t3s = L.field_min(S_T3S, sps, n);
   if (L.last_rc) return L.last_rc;
   // FP16 directions and an FP32 solution: iterations run in pairs with the
@@ -1288,6 +1308,16 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
         CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_BETA, sw, L.stream));
         return 0;
       }
+      if (dwf_rih) {
+        // M p with |M p|^2, alpha, then r -= alpha M^dag (M p) with |r|^2
+        int e = apply_m(L, op, hprec, t3s.ref(), pw, t1s.ref(), 0, part, nw);
+        if (e) return e;
+        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
+        e = dwf_mdag_rupdate(L, op, hprec, rsw, t3s.ref(), t1s.ref(), sw.alpha, sw.active, part, nw);
+        if (e) return e;
+        CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_BETA, sw, L.stream));
+        return 0;
+      }
       int e = apply_mdagm(L, op, hprec, apsw, pw, t1s.ref(), t2s.ref(), part, nw, s5_in_xpay);
       if (e) return e;
       CPSHIP_CHECK(reduce_rhs(part, bpr_hop, nw, nullptr, RED_CG_ALPHA, sw, L.stream));
@@ -1326,7 +1356,7 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
                                        bsb, L.stream));
       else
         CPSHIP_CHECK(blas_xpay_mixed(sps, psw, svec, rsw, sw.beta, sw.active, o.per_rhs, nw, bsb, L.stream));
-      if (clover_rih) return opstep(psw);
+      if (clover_rih || dwf_rih) return opstep(psw);
       if (r_in_hop) {
         // M p with |M p|^2, alpha, then r -= alpha M^dag (M p) with |r|^2
         int e = mobius_m(L, op, hprec, t3s.ref(), psw, t1s.ref(), t2s.ref(), 0, part, nw, true);

@@ -115,7 +115,7 @@ typedef struct {
   int lagged_x;         /* mixed CG: solution step applied by the next direction update */
   int paired_x;         /* mixed CG: two solution steps per pass over x (FP16 directions) */
   int s5_xpay;          /* mixed Moebius Ls 16: first fifth-dimension step of M in the direction update */
-  int r_in_hop;         /* mixed Moebius Ls 16: residual step in the last hop of M^dag (rounds differently) */
+  int r_in_hop;         /* mixed Moebius Ls 16 and DWF: residual step in the last hop of M^dag (rounds differently) */
   int clover_r_in_hop;  /* mixed clover: residual step in the last hop of MatPcDag (default 0: it spills) */
   int m5_fused;         /* half precision Ls 16: M5^-1 in the first hop's epilogue */
   int s5_fused;         /* Moebius M^dag: last fifth-dimension step in the hop epilogue */

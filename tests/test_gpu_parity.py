@@ -1064,6 +1064,43 @@ def test_clover_mixed_residual_step_in_hop(monkeypatch, paired):
     assert rel(out["1"][0], out["0"][0]) < 1e-8
 
 
+@pytest.mark.parametrize("paired", ["1", "0"])
+def test_dwf_mixed_residual_step_in_hop(oracle, monkeypatch, paired):
+    """Shamir DWF, half precision: r -= alpha M^dag M p and |r|^2 in the
+    epilogue of the second hop of M^dag (CLOV_NRR, the default; M^dag M p
+    never stored) against the separate pass (CPS_HIP_R_IN_HOP=0), with paired
+    and with single solution steps: the FP64 stopping rule, iteration counts
+    within 5%, and both solutions against the oracle's FP64 CG."""
+    monkeypatch.setenv("CPS_HIP_PAIRED_X", paired)
+    L, ls = [4, 4, 4, 8], 8
+    u = wilson_gauge(L, 0.2, 280)
+    nrhs = 3
+    vcb = int(np.prod(L)) // 2
+    n = 24 * vcb * ls
+    src = np.concatenate([F.gaussian_fermion(vcb * ls, 281 + k) * (1.0 + 2.0 * k) for k in range(nrhs)])
+    op = C.make_op(C.OP_DWF, mass=0.02, m5=1.8)
+    arg = C.make_cg_arg(1e-10, 3000, precision=C.PREC_MIXED_HALF, reliable_every=40)
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("CPS_HIP_R_IN_HOP", fused)
+        lat = C.Lattice(L, ls, (0, 0, 0, 1))
+        lat.import_gauge(u)
+        out[fused] = lat.invcg(op, src, arg, nrhs=nrhs)
+        lat.close()
+    it1 = [r.iterations for r in out["1"][1]]
+    it0 = [r.iterations for r in out["0"][1]]
+    print("dwf r in hop", it1, it0, flush=True)
+    for f in ("1", "0"):
+        assert max(r.true_res for r in out[f][1]) <= 1e-10
+        assert all(r.status == 1 for r in out[f][1])
+    for a, b_ in zip(it1, it0):
+        assert abs(a - b_) <= 0.05 * b_ + 2
+    d = oracle.dwf_struct(L, ls, 1.8)
+    for k in range(nrhs):
+        ref, _, _ = oracle.dwf_invcg(u, src[k * n:(k + 1) * n], 0.02, 1e-12, 5000, d)
+        assert rel(out["1"][0][k * n:(k + 1) * n], ref) < 1e-8
+
+
 @pytest.mark.parametrize("case", ["clover", "wilson"])
 def test_mixed_cg_mapping_consistent(monkeypatch, case):
     """A mixed-precision CG with the 4-D operators' hops free to use the
