@@ -390,7 +390,7 @@ def main():
                          "for hardware-counter runs")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {a.gpus}",
+        print(f"bench.py runs on one GPU only (the library has no multi-GPU path); got --gpus {a.gpus}",
               file=sys.stderr)
         sys.exit(2)
     if a.config == "c5":
