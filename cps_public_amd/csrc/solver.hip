@@ -619,9 +619,9 @@ int tuned_launch(Lattice& L, HopParams& P) {
           {
             // element order with the links shared by each row: only with
             // CPS_HIP_HOP_MODE=row (then the only candidate where possible);
-            // measured no faster than the element order at C3h (the M5^-1
-            // hop 4.47 against 3.63 ms, the s-term hop a tie), so the
-            // default search leaves it out
+            // measured no faster than the element order on the 24^3x64x16
+            // Moebius operator with 12 rhs (the M5^-1 hop 4.47 against 3.63
+            // ms, the s-term hop a tie), so the default search leaves it out
             HopParams Q = P;
             Q.flags = HOP_LINK_ROW;
             const bool row_ok = link_row_setup(Q, b);
@@ -1084,8 +1084,9 @@ void precision_formats(int prec, const cps_hip_op* op, int ls, int nrhs, bool re
 
 // Storage of the sloppy solution xs and residual rs with half-precision
 // sloppy spinors: xs FP32 and rs FP16 with per-site norms by default (the
-// same iteration counts as an FP32 rs at C2, C3h and C4, and 44 fewer bytes
-// per element each time rs is read or written); CPS_HIP_SLOPPY_VECTORS (read
+// same iteration counts as an FP32 rs on the 16^3x32x16 DWF, 24^3x64x16
+// Moebius and 32^3x64 clover solves, and 44 fewer bytes per element each
+// time rs is read or written); CPS_HIP_SLOPPY_VECTORS (read
 // per solve) "single" keeps both FP32, "xhalf" stores xs in FP16 and rs in
 // FP32, "half" both in FP16 (an FP16 xs triples the iteration count).
 static void sloppy_vector_storage(int sps, int* xst, int* rst) {
@@ -1230,8 +1231,8 @@ int cg_mixed_device(Lattice& L, const cps_hip_op* op, Field& x, Field& src, int 
   // (CPS_HIP_CLOVER_R_IN_HOP=1): likewise the residual step in the epilogue
   // of the last hop of MatPcDag, after the alpha of the first half.  Off by
   // default: that epilogue (clover multiply, X and r in registers) spills
-  // at the hop's register budget and the C4 solve takes 3.85 instead of
-  // 2.56 s.
+  // at the hop's register budget and the 32^3x64 clover solve with 48 rhs
+  // takes 3.85 instead of 2.56 s.
   const bool clover_rih = lagged && op->kind == CPS_HIP_OP_CLOVER && sps == STORE_H && svec == STORE_H &&
                           tn.clover_r_in_hop != 0;
   // DWF, FP16 directions and residual: the residual step in the epilogue of
