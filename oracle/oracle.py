@@ -15,7 +15,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# CPS_ORACLE_LIB: another build of the same sources (e.g. liboracle_asan.so)
+_LIB_PATH = os.environ.get("CPS_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 # the same sources built -O3 for x86-64-v3 with FP contraction: used only to
 # time the CPU baseline (results differ from liboracle.so by rounding)
 _FAST_PATH = os.path.join(_HERE, "liboracle_fast.so")
